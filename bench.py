@@ -1,0 +1,154 @@
+#!/usr/bin/env python3
+"""Benchmark: batched SAC singleton-test fixpoints per second on one MI355X.
+
+Workload (synthetic, seeded): random binary CSP n=500, d=256 (4 x 64-bit
+words), e=20000 distinct pairs, every tuple forbidden with p=0.5.  After the
+root GAC fixpoint, one step = one SAC pass = every singleton test (x = a) of
+the current domains (128,000 tests), each a full GAC fixpoint on its own
+domain copy, all in one persistent launch.  The plain path (one host-driven
+fixpoint per test, one launch + sync per round) and the CPU oracle are timed
+on bounded samples of the same tests in the same run.
+
+Prints one JSON line (see DESIGN.md for the roofline definitions).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "GAC fixpoints/sec batched vs one-per-launch on MI355X; HBM GB/s; p99 solve ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--plain-sample", type=int, default=400,
+                    help="singleton tests timed on the plain one-per-launch path")
+    ap.add_argument("--cpu-sample", type=int, default=600,
+                    help="singleton tests timed on the CPU oracle (about 10-15 s)")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is design-only, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    from cpim_amd import GModelAdapter, workload
+    from cpim_amd.model import domains_to_sets
+
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
+    t0 = time.time()
+    desc = workload.batched_sac()
+    n, W = desc.num_vars, desc.words
+    log(f"workload generated n={n} d={desc.max_dom} e={desc.num_cons} in {time.time()-t0:.1f}s")
+    m = GModelAdapter.Build(desc)
+    log(f"model on device: {m.device_bytes/2**20:.0f} MiB")
+    root = m.EnforceGAC_Auto()
+    if root.inconsistent:
+        raise SystemExit("root GAC wiped out; no singleton tests to run")
+    dom = m.ReadDomains(0)
+    tests = np.array([(v, a) for v, vals in enumerate(domains_to_sets(dom, desc.dom_size))
+                      for a in vals], np.int32)
+    ntests = len(tests)
+    # algorithmic bytes of one singleton test (x = a): the seed revisions
+    # t <- x read one support row and three domain word-sets each, plus the
+    # domain copy: 8 W (4 deg(x) + n)
+    deg = np.bincount(desc.con_scope, minlength=n)
+    bytes_per_pass = float(np.sum(8.0 * W * (4.0 * deg[tests[:, 0]] + n)))
+
+    for _ in range(args.warmup):
+        m.SingletonTests(tests)
+    kernel_ms = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        fail, st = m.SingletonTests(tests)
+        kernel_ms.append(st.kernel_ms)
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = ntests * args.steps / elapsed
+    avg_kernel_ms = float(np.mean(kernel_ms))
+    achieved = bytes_per_pass / (avg_kernel_ms * 1e-3) / 1e9
+    log(f"batched: {value:,.0f} fixpoints/s, {ms_per_step:.1f} ms/step, kernel {avg_kernel_ms:.1f} ms,"
+        f" {int(fail.sum())} failed tests")
+
+    plain = None
+    if not args.skip_plain:
+        k = min(args.plain_sample, ntests)
+        idx = np.linspace(0, ntests - 1, k).astype(np.int64)
+        m.SingletonTests(tests[idx[:8]], batched=False)
+        tp = time.perf_counter()
+        pf, _ = m.SingletonTests(tests[idx], batched=False)
+        plain = k / (time.perf_counter() - tp)
+        assert (pf == fail[idx]).all(), "plain and batched singleton tests disagree"
+        log(f"plain one-per-launch: {plain:,.0f} fixpoints/s on {k} tests")
+
+    cpu = None
+    if not args.skip_cpu:
+        import oracle
+        o = oracle.Oracle(desc)
+        o.root_ac()
+        k = min(args.cpu_sample, ntests)
+        idx = np.linspace(0, ntests - 1, k).astype(np.int64)
+        tc = time.perf_counter()
+        cf, _ = o.singleton_tests(tests[idx], nthreads=1)
+        cdt = time.perf_counter() - tc
+        assert (cf == fail[idx]).all(), "device and CPU oracle disagree"
+        cpu = {"value": k / cdt, "unit": "fixpoints/s", "cores": 1, "kind": "port",
+               "sample": f"{k} singleton tests of the same pass (AC3bit restatement, "
+                         f"{cdt:.1f} s single-threaded)"}
+        log(f"cpu oracle: {cpu['value']:.1f} fixpoints/s")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "fixpoints/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded random binary CSP)",
+        "config": {
+            "workload": "batched SAC1 singleton tests, random binary CSP n=500 d=256 e=20000 "
+                        "tightness 0.5, one SAC pass per step",
+            "singleton_tests_per_step": ntests,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+        "plain_fixpoints_per_s": plain,
+        "speedup_vs_plain": (value / plain) if plain else None,
+        "kernel_ms_per_step": avg_kernel_ms,
+        "revisions_per_s": st.revisions / (avg_kernel_ms * 1e-3),
+    }
+    m.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,231 @@
+// Device-side data layout and wave/workgroup propagation primitives (gfx950).
+//
+// Layout in HBM (all offsets in 64-bit words):
+//   dom  [level][n][W]          domain bitsets, LSB first (value v -> word v/64, bit v%64)
+//   size [level][n]             exact popcount of every domain, kept by atomics
+//   sup  [c][dir][w][dpad]      binary support planes, word-major: element (w, a) of
+//                               plane dir is the word w of the support row of value a
+//                               of scope[dir] (bits over scope[1-dir]'s domain).
+//                               dpad = 64 W.  Word-major rows make the per-value check
+//                               of a revision a coalesced 8-byte-per-lane load.
+//   tsup [c][pos][a][tw]        table constraints: bitset over tuple indices of the
+//                               tuples whose pos-th value is a.
+//   adj  [adj_off[v]..adj_off[v+1])  CSR of the constraints of v in constraint order:
+//                               {c, other var (binary) or -1 (table), pos of v, 0}
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef unsigned long long u64;
+
+#define CPIM_MAX_ARITY 4
+#define CPIM_MAX_WORDS 16   // domains up to 1024 values
+
+struct ConMeta {
+  int kind;  // 0 = binary, 1 = table
+  int arity;
+  int scope[CPIM_MAX_ARITY];
+  int tw;    // table: 64-bit words per tuple bitset
+  int pad;
+  long long off;  // binary: offset of plane 0 in sup; table: offset in tsup
+};
+
+struct DevNet {
+  int n;
+  int W;
+  int dpad;
+  int ncons;
+  const int* adj_off;
+  const int4* adj;
+  const ConMeta* con;
+  const u64* sup;
+  const u64* tsup;
+};
+
+// ---------------------------------------------------------------------------
+// Wave helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ u64 readlane64(u64 v, int lane) {
+  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)(v & 0xffffffffull), lane);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(v >> 32), lane);
+  return ((u64)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ int mbcnt(u64 mask) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+__device__ __forceinline__ u64 mix64(u64 z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// ---------------------------------------------------------------------------
+// Arc revision u <- t of a binary constraint, executed by one wave.
+//
+// Keeps value b of u iff some value of t supports it.  If t has one live value
+// a, the new D(u) is D(u) AND row_t[a] (W loads).  Otherwise every live value
+// of u checks its row against D(t), word by word with early exit (lanes = 64
+// values of u per chunk).  The result mask is applied with atomic AND, the
+// exact number of values this wave removed is subtracted from size[u], and the
+// wave that brings size[u] to 0 reports the wipe-out.  Concurrent revisions of
+// the same domain are safe: domains only shrink, a stale (larger) D(t) can only
+// keep too much, and every removal re-enqueues the constraints of u.
+// Returns the number of values removed (wave-uniform); *wiped is set when
+// this wave emptied D(u).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* size, int c, int u,
+                                             int t, int pos_t, bool* wiped) {
+  const int W = net.W;
+  const int dpad = net.dpad;
+  const int lane = lane_id();
+  const u64* base = net.sup + net.con[c].off;
+  const u64* plane_u = base + (long long)(1 - pos_t) * W * dpad;
+  const u64* plane_t = base + (long long)pos_t * W * dpad;
+  const u64 dt_l = lane < W ? dom[(long long)t * W + lane] : 0ull;
+  const u64 du_l = lane < W ? dom[(long long)u * W + lane] : 0ull;
+  int cnt_t = 0;
+  int first_w = -1;
+  for (int w = 0; w < W; ++w) {
+    const u64 x = readlane64(dt_l, w);
+    cnt_t += __popcll(x);
+    if (first_w < 0 && x) first_w = w;
+  }
+  u64 mask_l = 0ull;
+  if (cnt_t <= 1) {
+    if (cnt_t == 1) {
+      const int a = first_w * 64 + (int)__builtin_ctzll(readlane64(dt_l, first_w));
+      if (lane < W) mask_l = plane_t[(long long)lane * dpad + a];
+    }
+  } else {
+    for (int j = 0; j < W; ++j) {
+      const u64 duj = readlane64(du_l, j);
+      if (duj == 0ull) continue;
+      const int b = j * 64 + lane;
+      bool keep = false;
+      if ((duj >> lane) & 1ull) {
+        for (int w = 0; w < W; ++w) {
+          const u64 dtw = readlane64(dt_l, w);
+          if (plane_u[(long long)w * dpad + b] & dtw) {
+            keep = true;
+            break;
+          }
+        }
+      }
+      const u64 mj = __ballot(keep);
+      if (lane == j) mask_l = mj;
+    }
+  }
+  int removed_l = 0;
+  if (lane < W && (du_l & ~mask_l)) {
+    const u64 old = atomicAnd(&dom[(long long)u * W + lane], mask_l);
+    removed_l = __popcll(old & ~mask_l);
+  }
+  int removed = 0;
+  for (int w = 0; w < W; ++w) removed += __builtin_amdgcn_readlane(removed_l, w);
+  bool wipe = false;
+  if (removed > 0) {
+    int old_size = 0;
+    if (lane == 0) old_size = atomicSub(&size[u], removed);
+    old_size = __builtin_amdgcn_readfirstlane(old_size);
+    wipe = old_size == removed;
+  }
+  *wiped = wipe;
+  return removed;
+}
+
+// ---------------------------------------------------------------------------
+// Table constraint revision (positive table, bitset per (pos, value) over the
+// tuple indices).  valid = AND over positions of OR over live values of
+// tsup[pos][value]; value b of the variable at position p keeps iff
+// tsup[p][b] AND valid is not empty.  Positions whose variable is `skip_var`,
+// or is assigned (asg bitmap, may be NULL), are not revised.  *wiped is set if this wave
+// emptied a domain.  on_removed(var, count) is called (by the whole wave)
+// for every variable that lost values.  Requires tw <= 64 (4096 tuples).
+// ---------------------------------------------------------------------------
+template <typename OnRemoved>
+__device__ __forceinline__ void revise_table(const DevNet& net, u64* dom, int* size, int c,
+                                             int skip_var, const unsigned* asg,
+                                             OnRemoved on_removed, bool* wiped) {
+  const int W = net.W;
+  const int dpad = net.dpad;
+  const int lane = lane_id();
+  const ConMeta* mp = net.con + c;
+  const int arity = mp->arity;
+  const u64* ts = net.tsup + mp->off;
+  const int tw = mp->tw;
+  // valid tuples, word `lane` (lanes < tw)
+  u64 valid_l = lane < tw ? ~0ull : 0ull;
+  for (int p = 0; p < arity; ++p) {
+    const int v = mp->scope[p];
+    u64 acc = 0ull;
+    for (int j = 0; j < W; ++j) {
+      u64 word = dom[(long long)v * W + j];
+      word = readlane64(word, 0);
+      while (word) {
+        const int bit = (int)__builtin_ctzll(word);
+        word &= word - 1;
+        const int a = j * 64 + bit;
+        if (lane < tw) acc |= ts[((long long)p * dpad + a) * tw + lane];
+      }
+    }
+    valid_l &= acc;
+  }
+  bool any_wipe = false;
+  for (int p = 0; p < arity; ++p) {
+    const int y = mp->scope[p];
+    if (y == skip_var) continue;
+    if (asg && ((asg[y >> 5] >> (y & 31)) & 1u)) continue;
+    const u64 du_l = lane < W ? dom[(long long)y * W + lane] : 0ull;
+    u64 mask_l = 0ull;
+    for (int j = 0; j < W; ++j) {
+      const u64 duj = readlane64(du_l, j);
+      if (duj == 0ull) continue;
+      const int b = j * 64 + lane;
+      bool keep = false;
+      if ((duj >> lane) & 1ull) {
+        const u64* row = ts + ((long long)p * dpad + b) * tw;
+        for (int w = 0; w < tw; ++w) {
+          if (row[w] & readlane64(valid_l, w)) {
+            keep = true;
+            break;
+          }
+        }
+      }
+      const u64 mj = __ballot(keep);
+      if (lane == j) mask_l = mj;
+    }
+    int removed_l = 0;
+    if (lane < W && (du_l & ~mask_l)) {
+      const u64 old = atomicAnd(&dom[(long long)y * W + lane], mask_l);
+      removed_l = __popcll(old & ~mask_l);
+    }
+    int removed = 0;
+    for (int w = 0; w < W; ++w) removed += __builtin_amdgcn_readlane(removed_l, w);
+    if (removed > 0) {
+      int old_size = 0;
+      if (lane == 0) old_size = atomicSub(&size[y], removed);
+      old_size = __builtin_amdgcn_readfirstlane(old_size);
+      if (old_size == removed) any_wipe = true;
+      on_removed(y, removed);
+    }
+  }
+  *wiped = any_wipe;
+}

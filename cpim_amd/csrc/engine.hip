@@ -1,0 +1,1187 @@
+// C-ABI implementation of include/cpim_gpu.h for MI355X (gfx950).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cpim_gpu.h"
+#include "kernels.h"
+
+#define CPIM_VERSION "cpim_amd 0.1 (gfx950)"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+thread_local std::string g_last_error;
+
+struct CpimFailure {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw CpimFailure{code, msg}; }
+
+#define HIPC(expr)                                                                        \
+  do {                                                                                    \
+    hipError_t e__ = (expr);                                                              \
+    if (e__ != hipSuccess)                                                                \
+      fail(e__ == hipErrorOutOfMemory ? CPIM_ERR_OUT_OF_MEMORY : CPIM_ERR_HIP,            \
+           std::string(#expr) + ": " + hipGetErrorString(e__));                           \
+  } while (0)
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return CPIM_OK;
+  } catch (const CpimFailure& e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return CPIM_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return CPIM_ERR_RUNTIME;
+  }
+}
+
+#define REQUIRE(cond, msg) \
+  do {                     \
+    if (!(cond)) fail(CPIM_ERR_INVALID_ARGUMENT, msg); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Device properties (cached)
+// ---------------------------------------------------------------------------
+struct DevProps {
+  int cus = 0;
+  int max_lds = 0;
+  int coop = 0;
+  long long wall_khz = 0;
+  long long mem = 0;
+  std::string arch;
+};
+
+DevProps props_of(int dev) {
+  static std::mutex mu;
+  static std::vector<std::unique_ptr<DevProps>> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)cache.size() <= dev) cache.resize(dev + 1);
+  if (!cache[dev]) {
+    auto p = std::make_unique<DevProps>();
+    hipDeviceProp_t prop;
+    HIPC(hipGetDeviceProperties(&prop, dev));
+    p->cus = prop.multiProcessorCount;
+    p->max_lds = (int)prop.sharedMemPerBlock;
+    p->mem = (long long)prop.totalGlobalMem;
+    p->arch = prop.gcnArchName;
+    int coop = 0;
+    HIPC(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    p->coop = coop;
+    int khz = 0;
+    HIPC(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    p->wall_khz = khz;
+    cache[dev] = std::move(p);
+  }
+  return *cache[dev];
+}
+
+// LDS used by one workgroup-engine domain copy; kept below 64 KB so that
+// several workgroups share a CU.
+constexpr long long kLdsBudget = 60 * 1024;
+
+// MAC nodes one launch may run before it returns to the host.
+constexpr long long kNodesPerLaunch = 1 << 14;
+
+// ---------------------------------------------------------------------------
+// Model image: validated host-side layout of one network
+// ---------------------------------------------------------------------------
+struct ModelImage {
+  int n = 0, max_dom = 0, W = 0, dpad = 0;
+  int num_cons = 0;       // given
+  int num_active = 0;     // propagated
+  std::vector<int> cap;
+  std::vector<ConMeta> con;
+  std::vector<int> adj_off;
+  std::vector<int4> adj;
+  std::vector<int> bin_cons;          // active indices of binary constraints (plane build order)
+  std::vector<u64> rel_dense;         // owned dense rows when the desc has none
+  const u64* rel = nullptr;           // [nb][max_dom][rel_words]
+  int rel_words = 0;
+  std::vector<u64> tsup;
+  long long sup_words = 0;
+  std::vector<u64> init_dom;          // [n][W]
+};
+
+ModelImage make_image(const cpim_model_desc* d, bool skip_non_binary, bool own_rel = false) {
+  REQUIRE(d != nullptr, "model description is null");
+  REQUIRE(d->num_vars >= 1, "model has no variables");
+  REQUIRE(d->dom_size != nullptr, "dom_size is null");
+  REQUIRE(d->num_cons >= 0, "negative constraint count");
+  REQUIRE(d->num_cons == 0 || (d->con_arity && d->con_scope), "constraint arrays are null");
+  ModelImage im;
+  im.n = d->num_vars;
+  im.cap.assign(d->dom_size, d->dom_size + im.n);
+  for (int v = 0; v < im.n; ++v) {
+    REQUIRE(im.cap[v] >= 1, "domain of variable " + std::to_string(v) + " is empty");
+    REQUIRE(im.cap[v] <= 64 * CPIM_MAX_WORDS, "domain size above 1024 is not supported");
+    im.max_dom = std::max(im.max_dom, im.cap[v]);
+  }
+  im.W = (im.max_dom + 63) / 64;
+  im.dpad = 64 * im.W;
+  im.num_cons = d->num_cons;
+  const long long nw = (long long)im.n * im.W;
+  // scopes
+  std::vector<long long> scope_off(d->num_cons + 1, 0);
+  int nbin_total = 0;
+  for (int c = 0; c < d->num_cons; ++c) {
+    const int ar = d->con_arity[c];
+    REQUIRE(ar >= 2, "constraint " + std::to_string(c) + " has arity below 2");
+    scope_off[c + 1] = scope_off[c] + ar;
+    if (ar == 2) ++nbin_total;
+  }
+  for (long long i = 0; i < scope_off[d->num_cons]; ++i)
+    REQUIRE(d->con_scope[i] >= 0 && d->con_scope[i] < im.n, "scope names an unknown variable");
+  const bool any_nary = nbin_total != d->num_cons;
+  if (any_nary && !skip_non_binary) REQUIRE(d->tup_off && d->tuples, "tables need tup_off/tuples");
+  if (!d->bin_rel) REQUIRE(nbin_total == 0 || (d->tup_off && d->tuples), "binary relations missing");
+  if (d->bin_rel) REQUIRE(d->rel_words >= im.W, "rel_words is below ceil(max_dom/64)");
+  // active constraints
+  long long sup_off = 0, tsup_off = 0;
+  int bin_k = 0;
+  std::vector<int> bin_ordinal_of_active;
+  for (int c = 0; c < d->num_cons; ++c) {
+    const int ar = d->con_arity[c];
+    const int* sc = d->con_scope + scope_off[c];
+    if (ar == 2) {
+      REQUIRE(sc[0] != sc[1], "binary constraint " + std::to_string(c) + " repeats a variable");
+      ConMeta m{};
+      m.kind = 0;
+      m.arity = 2;
+      m.scope[0] = sc[0];
+      m.scope[1] = sc[1];
+      m.off = sup_off;
+      sup_off += 2ll * im.W * im.dpad;
+      im.bin_cons.push_back((int)im.con.size());
+      bin_ordinal_of_active.push_back(bin_k);
+      im.con.push_back(m);
+      ++bin_k;
+    } else {
+      if (skip_non_binary) continue;
+      REQUIRE(ar <= CPIM_MAX_ARITY, "tables of arity above 4 are not supported");
+      const long long t0 = d->tup_off[c], t1 = d->tup_off[c + 1];
+      REQUIRE(t1 >= t0 && (t1 - t0) % ar == 0, "tup_off does not delimit whole tuples");
+      const long long T = (t1 - t0) / ar;
+      REQUIRE(T <= 64 * 64, "tables above 4096 tuples are not supported");
+      for (int i = 0; i < ar; ++i)
+        for (int j = i + 1; j < ar; ++j) REQUIRE(sc[i] != sc[j], "table scope repeats a variable");
+      ConMeta m{};
+      m.kind = 1;
+      m.arity = ar;
+      for (int i = 0; i < ar; ++i) m.scope[i] = sc[i];
+      m.tw = std::max<int>(1, (int)((T + 63) / 64));
+      m.off = tsup_off;
+      const long long words = (long long)ar * im.dpad * m.tw;
+      im.tsup.resize(tsup_off + words, 0ull);
+      for (long long t = 0; t < T; ++t) {
+        const int32_t* tup = d->tuples + t0 + t * ar;
+        bool ok = true;
+        for (int i = 0; i < ar; ++i) ok = ok && tup[i] >= 0 && tup[i] < im.cap[sc[i]];
+        REQUIRE(ok, "table tuple value outside its domain");
+        for (int i = 0; i < ar; ++i)
+          im.tsup[tsup_off + ((long long)i * im.dpad + tup[i]) * m.tw + (t >> 6)] |= 1ull << (t & 63);
+      }
+      tsup_off += words;
+      bin_ordinal_of_active.push_back(-1);
+      im.con.push_back(m);
+    }
+  }
+  im.num_active = (int)im.con.size();
+  im.sup_words = std::max<long long>(sup_off, 1);
+  if (im.tsup.empty()) im.tsup.assign(1, 0ull);
+  // dense binary rows
+  if (d->bin_rel) {
+    im.rel = reinterpret_cast<const u64*>(d->bin_rel);
+    im.rel_words = d->rel_words;
+    if (own_rel) {
+      im.rel_dense.assign(im.rel, im.rel + (size_t)nbin_total * im.max_dom * d->rel_words);
+      im.rel = im.rel_dense.data();
+    }
+  } else if (!im.bin_cons.empty()) {
+    im.rel_words = im.W;
+    im.rel_dense.assign((size_t)im.bin_cons.size() * im.max_dom * im.W, 0ull);
+    int k = 0;
+    for (int c = 0; c < d->num_cons; ++c) {
+      if (d->con_arity[c] != 2) continue;
+      const int* sc = d->con_scope + scope_off[c];
+      REQUIRE(d->tup_off[c + 1] >= d->tup_off[c] && (d->tup_off[c + 1] - d->tup_off[c]) % 2 == 0,
+              "tup_off does not delimit whole tuples");
+      for (long long t = d->tup_off[c]; t < d->tup_off[c + 1]; t += 2) {
+        const int a = d->tuples[t], b = d->tuples[t + 1];
+        REQUIRE(a >= 0 && a < im.cap[sc[0]] && b >= 0 && b < im.cap[sc[1]],
+                "binary tuple value outside its domain");
+        im.rel_dense[((size_t)k * im.max_dom + a) * im.W + (b >> 6)] |= 1ull << (b & 63);
+      }
+      ++k;
+    }
+    im.rel = im.rel_dense.data();
+  }
+  // CSR (constraint order per variable, like Network's subscription lists)
+  std::vector<int> deg(im.n, 0);
+  for (const auto& m : im.con)
+    for (int i = 0; i < m.arity; ++i) ++deg[m.scope[i]];
+  im.adj_off.assign(im.n + 1, 0);
+  for (int v = 0; v < im.n; ++v) im.adj_off[v + 1] = im.adj_off[v] + deg[v];
+  im.adj.resize(im.adj_off[im.n]);
+  std::vector<int> pos(im.adj_off.begin(), im.adj_off.end() - 1);
+  for (int c = 0; c < im.num_active; ++c) {
+    const auto& m = im.con[c];
+    for (int i = 0; i < m.arity; ++i) {
+      const int v = m.scope[i];
+      const int other = m.kind == 0 ? m.scope[1 - i] : -1;
+      im.adj[pos[v]++] = make_int4(c, other, i, 0);
+    }
+  }
+  // level 0
+  im.init_dom.assign(nw, 0ull);
+  for (int v = 0; v < im.n; ++v)
+    for (int a = 0; a < im.cap[v]; ++a) {
+      const bool keep = d->init_dom == nullptr ||
+                        ((d->init_dom[(long long)v * im.W + (a >> 6)] >> (a & 63)) & 1ull);
+      if (keep) im.init_dom[(long long)v * im.W + (a >> 6)] |= 1ull << (a & 63);
+    }
+  return im;
+}
+
+// ---------------------------------------------------------------------------
+// Device copy of a network
+// ---------------------------------------------------------------------------
+struct DevAlloc {
+  void* p = nullptr;
+  DevAlloc() = default;
+  explicit DevAlloc(size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    HIPC(hipMalloc(&p, bytes));
+  }
+  ~DevAlloc() {
+    if (p) (void)hipFree(p);
+  }
+  DevAlloc(const DevAlloc&) = delete;
+  DevAlloc& operator=(const DevAlloc&) = delete;
+  DevAlloc(DevAlloc&& o) noexcept : p(o.p) { o.p = nullptr; }
+  DevAlloc& operator=(DevAlloc&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      o.p = nullptr;
+    }
+    return *this;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+struct DevModel {
+  int n = 0, W = 0, dpad = 0, max_dom = 0, num_active = 0;
+  DevAlloc adj_off, adj, con, sup, tsup, cap;
+  long long bytes = 0;
+  DevNet net() const {
+    DevNet d;
+    d.n = n;
+    d.W = W;
+    d.dpad = dpad;
+    d.ncons = num_active;
+    d.adj_off = adj_off.as<int>();
+    d.adj = adj.as<int4>();
+    d.con = con.as<ConMeta>();
+    d.sup = sup.as<u64>();
+    d.tsup = tsup.as<u64>();
+    return d;
+  }
+};
+
+template <typename T>
+void upload(DevAlloc& dst, const std::vector<T>& src, hipStream_t s, long long& bytes) {
+  const size_t b = std::max<size_t>(sizeof(T) * src.size(), 16);
+  dst = DevAlloc(b);
+  if (!src.empty()) HIPC(hipMemcpyAsync(dst.p, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, s));
+  bytes += (long long)b;
+}
+
+void build_device_model(const ModelImage& im, DevModel& dm, hipStream_t s) {
+  dm.n = im.n;
+  dm.W = im.W;
+  dm.dpad = im.dpad;
+  dm.max_dom = im.max_dom;
+  dm.num_active = im.num_active;
+  upload(dm.adj_off, im.adj_off, s, dm.bytes);
+  upload(dm.adj, im.adj, s, dm.bytes);
+  upload(dm.con, im.con, s, dm.bytes);
+  upload(dm.tsup, im.tsup, s, dm.bytes);
+  upload(dm.cap, im.cap, s, dm.bytes);
+  dm.sup = DevAlloc(sizeof(u64) * im.sup_words);
+  dm.bytes += sizeof(u64) * im.sup_words;
+  const int nb = (int)im.bin_cons.size();
+  if (nb > 0) {
+    // Upload the dense rows in slices and build the planes on the device.
+    DevAlloc bins;
+    long long dummy = 0;
+    upload(bins, im.bin_cons, s, dummy);
+    const long long row_words = (long long)im.max_dom * im.rel_words;
+    const int slice = (int)std::max<long long>(1, std::min<long long>(nb, (256ll << 20) / (8 * row_words)));
+    DevAlloc rel_dev(sizeof(u64) * row_words * slice);
+    for (int k0 = 0; k0 < nb; k0 += slice) {
+      const int k1 = std::min(nb, k0 + slice);
+      HIPC(hipMemcpyAsync(rel_dev.p, im.rel + (long long)k0 * row_words,
+                          sizeof(u64) * row_words * (k1 - k0), hipMemcpyHostToDevice, s));
+      const long long waves = (long long)(k1 - k0) * im.W;
+      const int threads = 256;
+      const long long blocks = (waves * 64 + threads - 1) / threads;
+      hipLaunchKernelGGL(k_build_planes, dim3((unsigned)blocks), dim3(threads), 0, s,
+                         rel_dev.as<u64>(), k1 - k0, im.max_dom, im.rel_words, im.W, im.dpad,
+                         bins.as<int>() + k0, dm.con.as<ConMeta>(), dm.cap.as<int>(), dm.sup.as<u64>());
+      HIPC(hipGetLastError());
+      HIPC(hipStreamSynchronize(s));
+    }
+  }
+  HIPC(hipStreamSynchronize(s));
+}
+
+struct DeviceGuard {
+  int prev = 0;
+  explicit DeviceGuard(int dev) {
+    HIPC(hipGetDevice(&prev));
+    if (prev != dev) HIPC(hipSetDevice(dev));
+  }
+  ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+long long now_ns() {
+  return std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// GModel
+// ---------------------------------------------------------------------------
+struct cpim_gmodel {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  ModelImage im;  // host layout (rel pointer not kept beyond build)
+  DevModel dm;
+  int max_depth = 0;
+  int current_level = 0;
+  int assigned_size = 0;
+  DevAlloc dom;    // [max_depth][n][W]
+  DevAlloc size;   // [max_depth][n]
+  DevAlloc front;  // 3 x nfw
+  DevAlloc ctl;    // GacCtl
+  DevAlloc block_out;
+  DevAlloc scratch;  // workgroup-engine scratch when a copy does not fit in LDS
+  long long scratch_bytes = 0;
+  int nfw = 0;
+  std::vector<int2> assigned;  // d_assigned mirror: (var, value) per level
+  DevProps props;
+
+  u64* level_dom(int level) const { return dom.as<u64>() + (long long)level * dm.n * dm.W; }
+  int* level_size(int level) const { return size.as<int>() + (long long)level * dm.n; }
+  long long level_words() const { return (long long)dm.n * dm.W; }
+
+  char* get_scratch(long long bytes) {
+    if (bytes > scratch_bytes) {
+      scratch = DevAlloc((size_t)bytes);
+      scratch_bytes = bytes;
+    }
+    return scratch.as<char>();
+  }
+  void sync() { HIPC(hipStreamSynchronize(stream)); }
+};
+
+extern "C" {
+
+void cpim_gmodel_options_default(cpim_gmodel_options* o) {
+  if (!o) return;
+  o->device_id = 0;
+  o->enable_prefetch = 0;
+  o->skip_non_binary = 1;
+  o->skip_conflicts = 0;
+}
+
+const char* cpim_last_error(void) { return g_last_error.c_str(); }
+const char* cpim_version(void) { return CPIM_VERSION; }
+
+int cpim_get_device_info(int32_t device_id, cpim_device_info* out) {
+  return guarded([&] {
+    REQUIRE(out != nullptr, "out is null");
+    int count = 0;
+    HIPC(hipGetDeviceCount(&count));
+    REQUIRE(device_id >= 0 && device_id < count, "invalid device id");
+    const DevProps p = props_of(device_id);
+    out->compute_units = p.cus;
+    out->lds_bytes_per_block = p.max_lds;
+    out->cooperative_launch = p.coop;
+    out->wavefront_size = 64;
+    out->global_mem_bytes = p.mem;
+    std::snprintf(out->arch, sizeof(out->arch), "%s", p.arch.c_str());
+  });
+}
+
+int cpim_gmodel_create(const cpim_model_desc* desc, const cpim_gmodel_options* opts,
+                       cpim_gmodel** out) {
+  return guarded([&] {
+    REQUIRE(out != nullptr, "out is null");
+    *out = nullptr;
+    cpim_gmodel_options o;
+    cpim_gmodel_options_default(&o);
+    if (opts) o = *opts;
+    // validate the model before touching the device
+    ModelImage im = make_image(desc, o.skip_non_binary != 0);
+    int count = 0;
+    HIPC(hipGetDeviceCount(&count));
+    REQUIRE(o.device_id >= 0 && o.device_id < count, "invalid device id");
+    DeviceGuard g(o.device_id);
+    auto m = std::make_unique<cpim_gmodel>();
+    m->device = o.device_id;
+    m->props = props_of(o.device_id);
+    HIPC(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    m->im = std::move(im);
+    build_device_model(m->im, m->dm, m->stream);
+    m->im.rel = nullptr;
+    m->im.rel_dense.clear();
+    m->im.rel_dense.shrink_to_fit();
+    const int n = m->dm.n;
+    m->max_depth = n + 1;
+    const long long lw = m->level_words();
+    m->dom = DevAlloc(sizeof(u64) * lw * m->max_depth);
+    m->size = DevAlloc(sizeof(int) * (long long)n * m->max_depth);
+    HIPC(hipMemcpyAsync(m->dom.p, m->im.init_dom.data(), sizeof(u64) * lw, hipMemcpyHostToDevice, m->stream));
+    hipLaunchKernelGGL(k_sizes, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->level_dom(0),
+                       m->level_size(0), n, m->dm.W);
+    HIPC(hipGetLastError());
+    m->nfw = std::max(1, (m->dm.num_active + 31) / 32);
+    m->front = DevAlloc(sizeof(unsigned) * 3ll * m->nfw);
+    m->ctl = DevAlloc(sizeof(GacCtl));
+    m->block_out = DevAlloc(sizeof(BlockGacOut));
+    m->assigned.assign(m->max_depth, make_int2(-1, -1));
+    m->dm.bytes += sizeof(u64) * lw * m->max_depth + sizeof(int) * (long long)n * m->max_depth +
+                   sizeof(unsigned) * 3ll * m->nfw + sizeof(GacCtl);
+    m->sync();
+    *out = m.release();
+  });
+}
+
+int cpim_gmodel_destroy(cpim_gmodel* m) {
+  return guarded([&] {
+    if (!m) return;
+    DeviceGuard g(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    hipStream_t s = m->stream;
+    delete m;
+    if (s) (void)hipStreamDestroy(s);
+  });
+}
+
+int cpim_gmodel_get_info(const cpim_gmodel* m, cpim_gmodel_info* out) {
+  return guarded([&] {
+    REQUIRE(m && out, "null argument");
+    out->num_vars = m->dm.n;
+    out->num_constraints = m->im.num_cons;
+    out->num_active = m->dm.num_active;
+    out->max_dom_size = m->dm.max_dom;
+    out->dom_words = m->dm.W;
+    out->bit_dom_int_size = (m->dm.max_dom + 31) / 32;
+    out->max_depth = m->max_depth;
+    out->subscription_size = m->im.adj_off.empty() ? 0 : m->im.adj_off.back();
+    out->device_bytes = m->dm.bytes;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// GAC on the current level
+// ---------------------------------------------------------------------------
+namespace {
+
+bool fits_lds(const cpim_gmodel* m) { return wg_mem_bytes(m->dm.n, m->dm.W) <= kLdsBudget; }
+
+void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats* st) {
+  const int L = m->current_level;
+  u64* dom = m->level_dom(L);
+  int* size = m->level_size(L);
+  unsigned* F = m->front.as<unsigned>();
+  GacCtl* ctl = m->ctl.as<GacCtl>();
+  const int nfw = m->nfw;
+  GacCtl init{};
+  init.last_push_round = -1;
+  init.wipe_round = INT_MAX;
+  HIPC(hipMemcpyAsync(ctl, &init, sizeof(init), hipMemcpyHostToDevice, m->stream));
+  HIPC(hipMemsetAsync(F, 0, sizeof(unsigned) * 3ll * nfw, m->stream));
+  if (assigned_var < 0) {
+    hipLaunchKernelGGL(k_seed_all, dim3((nfw + 255) / 256), dim3(256), 0, m->stream, F, nfw,
+                       m->dm.num_active);
+  } else {
+    hipLaunchKernelGGL(k_seed_var, dim3(1), dim3(256), 0, m->stream, F, m->dm.adj_off.as<int>(),
+                       m->dm.adj.as<int4>(), assigned_var);
+  }
+  HIPC(hipGetLastError());
+  DevNet net = m->dm.net();
+  const int threads = 256;
+  const long long max_rounds = (long long)m->dm.n * m->dm.max_dom + 2;
+  int rounds = 0;
+  GacCtl res{};
+  if (persistent) {
+    int per_cu = 0;
+    HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gac_persistent, threads, 0));
+    per_cu = std::max(1, std::min(per_cu, 4));
+    const int waves_per_block = threads / 64;
+    const int want = std::max(1, (nfw + waves_per_block - 1) / waves_per_block);
+    const int grid = std::max(1, std::min(want, per_cu * m->props.cus));
+    int mr = (int)std::min<long long>(max_rounds, INT_MAX);
+    void* args[] = {&net, &dom, &size, &F, (void*)&nfw, &ctl, &mr};
+    HIPC(hipLaunchCooperativeKernel((const void*)k_gac_persistent, dim3(grid), dim3(threads), args, 0,
+                                    m->stream));
+    HIPC(hipMemcpyAsync(&res, ctl, sizeof(res), hipMemcpyDeviceToHost, m->stream));
+    m->sync();
+    rounds = res.rounds;
+  } else {
+    const int waves_per_block = threads / 64;
+    const int grid = std::max(1, std::min((nfw + waves_per_block - 1) / waves_per_block, 8 * m->props.cus));
+    for (int r = 0; r < max_rounds; ++r) {
+      hipLaunchKernelGGL(k_gac_round, dim3(grid), dim3(threads), 0, m->stream, net, dom, size, F, nfw,
+                         ctl, r);
+      HIPC(hipGetLastError());
+      HIPC(hipMemcpyAsync(&res, ctl, sizeof(res), hipMemcpyDeviceToHost, m->stream));
+      m->sync();
+      rounds = r + 1;
+      if (res.wipe_round <= r) break;
+      if (res.last_push_round < r) {
+        res.converged = 1;
+        break;
+      }
+    }
+  }
+  st->iterations = rounds;
+  st->deletions = (int32_t)res.deletions;
+  st->inconsistent = res.wipe_round != INT_MAX;
+  st->converged = st->inconsistent ? 0 : res.converged;
+  st->revisions = (int64_t)res.revisions;
+}
+
+void gac_block(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
+  const int L = m->current_level;
+  DevNet net = m->dm.net();
+  const long long bytes = wg_mem_bytes(m->dm.n, m->dm.W);
+  const int threads = 1024;
+  BlockGacOut* out = m->block_out.as<BlockGacOut>();
+  if (bytes <= kLdsBudget) {
+    hipLaunchKernelGGL(k_block_gac<true>, dim3(1), dim3(threads), (size_t)bytes, m->stream, net,
+                       m->level_dom(L), m->level_size(L), assigned_var, (char*)nullptr, out);
+  } else {
+    char* scratch = m->get_scratch(bytes);
+    hipLaunchKernelGGL(k_block_gac<false>, dim3(1), dim3(threads), 0, m->stream, net, m->level_dom(L),
+                       m->level_size(L), assigned_var, scratch, out);
+  }
+  HIPC(hipGetLastError());
+  BlockGacOut res{};
+  HIPC(hipMemcpyAsync(&res, out, sizeof(res), hipMemcpyDeviceToHost, m->stream));
+  m->sync();
+  st->iterations = 1;
+  st->deletions = (int32_t)res.deletions;
+  st->inconsistent = !res.consistent;
+  st->converged = res.consistent;
+  st->revisions = (int64_t)res.revisions;
+}
+
+int auto_gac_mode(const cpim_gmodel* m) {
+  // Small networks: one workgroup with the domains in LDS (one launch, no
+  // grid barrier).  Large ones: the persistent grid when cooperative launch
+  // is available.
+  if (fits_lds(m) && m->dm.num_active <= 4096) return CPIM_GAC_BLOCK;
+  return m->props.coop ? CPIM_GAC_PERSISTENT : CPIM_GAC_PLAIN;
+}
+
+void gac(cpim_gmodel* m, int assigned_var, int mode, cpim_gac_stats* st) {
+  std::memset(st, 0, sizeof(*st));
+  if (mode == CPIM_GAC_AUTO) mode = auto_gac_mode(m);
+  if (mode == CPIM_GAC_PERSISTENT && !m->props.coop) mode = CPIM_GAC_PLAIN;  // reference fallback
+  if (m->dm.num_active == 0) {
+    st->converged = 1;
+    return;
+  }
+  switch (mode) {
+    case CPIM_GAC_PLAIN: gac_grid(m, assigned_var, false, st); break;
+    case CPIM_GAC_PERSISTENT: gac_grid(m, assigned_var, true, st); break;
+    case CPIM_GAC_BLOCK: gac_block(m, assigned_var, st); break;
+    default: fail(CPIM_ERR_INVALID_ARGUMENT, "unknown GAC mode " + std::to_string(mode));
+  }
+}
+
+void check_var(const cpim_gmodel* m, int var, const char* fn) {
+  if (var < 0 || var >= m->dm.n)
+    fail(CPIM_ERR_RUNTIME, std::string("[GModel::") + fn + "] Invalid var: " + std::to_string(var));
+}
+
+void check_level(const cpim_gmodel* m, int level, const char* fn) {
+  if (level < 0 || level >= m->max_depth)
+    fail(CPIM_ERR_RUNTIME, std::string("[GModel::") + fn + "] Invalid level: " + std::to_string(level));
+}
+
+std::vector<u64> read_var(const cpim_gmodel* m, int var, int level) {
+  std::vector<u64> w(m->dm.W);
+  HIPC(hipMemcpyAsync(w.data(), m->level_dom(level) + (long long)var * m->dm.W, sizeof(u64) * m->dm.W,
+                      hipMemcpyDeviceToHost, m->stream));
+  HIPC(hipStreamSynchronize(m->stream));
+  return w;
+}
+
+int read_size(const cpim_gmodel* m, int var, int level) {
+  int s = 0;
+  HIPC(hipMemcpyAsync(&s, m->level_size(level) + var, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+  HIPC(hipStreamSynchronize(m->stream));
+  return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpim_gmodel_enforce_gac(cpim_gmodel* m, int32_t assigned_var, int32_t mode, cpim_gac_stats* st) {
+  return guarded([&] {
+    REQUIRE(m && st, "null argument");
+    if (assigned_var >= 0) check_var(m, assigned_var, "EnforceGAC");
+    DeviceGuard g(m->device);
+    gac(m, assigned_var, mode, st);
+  });
+}
+
+int cpim_gmodel_create_new_level(cpim_gmodel* m, int32_t* new_level) {
+  return guarded([&] {
+    REQUIRE(m && new_level, "null argument");
+    if (m->current_level >= m->max_depth - 1)
+      fail(CPIM_ERR_RUNTIME, "[GModel::CreateNewLevel] Max depth reached: " + std::to_string(m->max_depth));
+    DeviceGuard g(m->device);
+    const int L = ++m->current_level;
+    HIPC(hipMemcpyAsync(m->level_dom(L), m->level_dom(L - 1), sizeof(u64) * m->level_words(),
+                        hipMemcpyDeviceToDevice, m->stream));
+    HIPC(hipMemcpyAsync(m->level_size(L), m->level_size(L - 1), sizeof(int) * m->dm.n,
+                        hipMemcpyDeviceToDevice, m->stream));
+    m->sync();
+    *new_level = L;
+  });
+}
+
+int cpim_gmodel_back_to_level(cpim_gmodel* m, int32_t level) {
+  return guarded([&] {
+    REQUIRE(m, "null argument");
+    if (level < 0 || level >= m->max_depth)
+      fail(CPIM_ERR_RUNTIME, "[GModel::BackToLevel] Invalid level: " + std::to_string(level));
+    if (level > m->current_level)
+      fail(CPIM_ERR_RUNTIME, "[GModel::BackToLevel] Cannot back to future level: " + std::to_string(level) +
+                                 " (current=" + std::to_string(m->current_level) + ")");
+    m->current_level = level;
+    while (m->assigned_size > level) --m->assigned_size;
+  });
+}
+
+int cpim_gmodel_current_level(const cpim_gmodel* m, int32_t* level) {
+  return guarded([&] {
+    REQUIRE(m && level, "null argument");
+    *level = m->current_level;
+  });
+}
+
+int cpim_gmodel_assign_value(cpim_gmodel* m, int32_t var, int32_t value, int32_t level, int32_t* ok) {
+  return guarded([&] {
+    REQUIRE(m && ok, "null argument");
+    check_var(m, var, "AssignValue");
+    check_level(m, level, "AssignValue");
+    if (value < 0 || value >= m->dm.max_dom)
+      fail(CPIM_ERR_RUNTIME, "[GModel::AssignValue] Invalid value: " + std::to_string(value));
+    DeviceGuard g(m->device);
+    std::vector<u64> w(m->dm.W, 0ull);
+    w[value >> 6] = 1ull << (value & 63);
+    const int one = 1;
+    HIPC(hipMemcpyAsync(m->level_dom(level) + (long long)var * m->dm.W, w.data(), sizeof(u64) * m->dm.W,
+                        hipMemcpyHostToDevice, m->stream));
+    HIPC(hipMemcpyAsync(m->level_size(level) + var, &one, sizeof(int), hipMemcpyHostToDevice, m->stream));
+    m->sync();
+    m->assigned[level] = make_int2(var, value);
+    m->assigned_size = std::max(m->assigned_size, level + 1);
+    *ok = 1;
+  });
+}
+
+int cpim_gmodel_remove_value(cpim_gmodel* m, int32_t var, int32_t value, int32_t level, int32_t* ok) {
+  return guarded([&] {
+    REQUIRE(m && ok, "null argument");
+    check_var(m, var, "RemoveValue");
+    check_level(m, level, "RemoveValue");
+    *ok = 0;
+    if (value < 0 || value >= m->dm.max_dom) return;
+    DeviceGuard g(m->device);
+    std::vector<u64> w = read_var(m, var, level);
+    const u64 bit = 1ull << (value & 63);
+    if (!(w[value >> 6] & bit)) return;
+    w[value >> 6] &= ~bit;
+    int s = 0;
+    for (u64 x : w) s += __builtin_popcountll(x);
+    HIPC(hipMemcpyAsync(m->level_dom(level) + (long long)var * m->dm.W + (value >> 6), &w[value >> 6],
+                        sizeof(u64), hipMemcpyHostToDevice, m->stream));
+    HIPC(hipMemcpyAsync(m->level_size(level) + var, &s, sizeof(int), hipMemcpyHostToDevice, m->stream));
+    m->sync();
+    *ok = s > 0;
+  });
+}
+
+int cpim_gmodel_domain_size(const cpim_gmodel* m, int32_t var, int32_t level, int32_t* size) {
+  return guarded([&] {
+    REQUIRE(m && size, "null argument");
+    check_var(m, var, "GetDomainSize");
+    check_level(m, level, "GetDomainSize");
+    DeviceGuard g(m->device);
+    *size = read_size(m, var, level);
+  });
+}
+
+int cpim_gmodel_min_domain_var(const cpim_gmodel* m, int32_t level, int32_t* var) {
+  return guarded([&] {
+    REQUIRE(m && var, "null argument");
+    check_level(m, level, "GetMinDomainVar");
+    DeviceGuard g(m->device);
+    std::vector<int> s(m->dm.n);
+    HIPC(hipMemcpyAsync(s.data(), m->level_size(level), sizeof(int) * m->dm.n, hipMemcpyDeviceToHost, m->stream));
+    HIPC(hipStreamSynchronize(m->stream));
+    int best = -1, best_size = m->dm.max_dom + 1;
+    for (int v = 0; v < m->dm.n; ++v)
+      if (s[v] > 1 && s[v] < best_size) {
+        best_size = s[v];
+        best = v;
+      }
+    *var = best;
+  });
+}
+
+int cpim_gmodel_is_assigned(const cpim_gmodel* m, int32_t var, int32_t level, int32_t* res) {
+  return guarded([&] {
+    REQUIRE(m && res, "null argument");
+    check_var(m, var, "IsAssigned");
+    check_level(m, level, "IsAssigned");
+    DeviceGuard g(m->device);
+    *res = read_size(m, var, level) == 1;
+  });
+}
+
+int cpim_gmodel_is_fully_assigned(const cpim_gmodel* m, int32_t level, int32_t* res) {
+  return guarded([&] {
+    REQUIRE(m && res, "null argument");
+    check_level(m, level, "IsFullyAssigned");
+    DeviceGuard g(m->device);
+    std::vector<int> s(m->dm.n);
+    HIPC(hipMemcpyAsync(s.data(), m->level_size(level), sizeof(int) * m->dm.n, hipMemcpyDeviceToHost, m->stream));
+    HIPC(hipStreamSynchronize(m->stream));
+    *res = std::all_of(s.begin(), s.end(), [](int x) { return x == 1; });
+  });
+}
+
+static int first_from(const std::vector<u64>& w, int start, int max_dom) {
+  for (int a = start; a < 64 * (int)w.size(); ++a) {
+    const u64 word = w[a >> 6] >> (a & 63);
+    if (word) {
+      const int v = a + (int)__builtin_ctzll(word);
+      return v < max_dom ? v : -1;
+    }
+    a |= 63;
+  }
+  return -1;
+}
+
+int cpim_gmodel_assigned_value(const cpim_gmodel* m, int32_t var, int32_t level, int32_t* value) {
+  return guarded([&] {
+    REQUIRE(m && value, "null argument");
+    check_var(m, var, "GetAssignedValue");
+    check_level(m, level, "GetAssignedValue");
+    DeviceGuard g(m->device);
+    if (read_size(m, var, level) != 1) {
+      *value = -1;
+      return;
+    }
+    *value = first_from(read_var(m, var, level), 0, m->dm.max_dom);
+  });
+}
+
+int cpim_gmodel_first_value(const cpim_gmodel* m, int32_t var, int32_t level, int32_t* value) {
+  return guarded([&] {
+    REQUIRE(m && value, "null argument");
+    check_var(m, var, "GetFirstValue");
+    check_level(m, level, "GetFirstValue");
+    DeviceGuard g(m->device);
+    *value = first_from(read_var(m, var, level), 0, m->dm.max_dom);
+  });
+}
+
+int cpim_gmodel_next_value(const cpim_gmodel* m, int32_t var, int32_t value, int32_t level, int32_t* next) {
+  return guarded([&] {
+    REQUIRE(m && next, "null argument");
+    check_var(m, var, "GetNextValue");
+    check_level(m, level, "GetNextValue");
+    if (value < 0 || value >= m->dm.max_dom - 1) {
+      *next = -1;
+      return;
+    }
+    DeviceGuard g(m->device);
+    *next = first_from(read_var(m, var, level), value + 1, m->dm.max_dom);
+  });
+}
+
+int cpim_gmodel_read_domains(const cpim_gmodel* m, int32_t level, uint64_t* out) {
+  return guarded([&] {
+    REQUIRE(m && out, "null argument");
+    check_level(m, level, "ReadDomains");
+    DeviceGuard g(m->device);
+    HIPC(hipMemcpyAsync(out, m->level_dom(level), sizeof(u64) * m->level_words(), hipMemcpyDeviceToHost,
+                        m->stream));
+    HIPC(hipStreamSynchronize(m->stream));
+  });
+}
+
+int cpim_gmodel_write_domains(cpim_gmodel* m, int32_t level, const uint64_t* in) {
+  return guarded([&] {
+    REQUIRE(m && in, "null argument");
+    check_level(m, level, "WriteDomains");
+    DeviceGuard g(m->device);
+    // mask to the domain capacities so that sizes stay exact
+    const u64* inw = reinterpret_cast<const u64*>(in);
+    std::vector<u64> w(inw, inw + m->level_words());
+    for (int v = 0; v < m->dm.n; ++v)
+      for (int j = 0; j < m->dm.W; ++j) {
+        const int k = m->im.cap[v] - 64 * j;
+        const u64 mask = k >= 64 ? ~0ull : (k <= 0 ? 0ull : ((1ull << k) - 1ull));
+        w[(long long)v * m->dm.W + j] &= mask;
+      }
+    HIPC(hipMemcpyAsync(m->level_dom(level), w.data(), sizeof(u64) * w.size(), hipMemcpyHostToDevice, m->stream));
+    hipLaunchKernelGGL(k_sizes, dim3((m->dm.n + 255) / 256), dim3(256), 0, m->stream, m->level_dom(level),
+                       m->level_size(level), m->dm.n, m->dm.W);
+    HIPC(hipGetLastError());
+    m->sync();
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Singleton tests and SAC
+// ---------------------------------------------------------------------------
+namespace {
+
+struct SacScratch {
+  DevAlloc tests, fail, counter, revs;
+  long long cap = 0;
+};
+
+void run_singleton_batched(cpim_gmodel* m, const int2* tests_host, long long count, unsigned char* fail_host,
+                           cpim_sac_stats* st, SacScratch& S) {
+  if (count == 0) return;
+  if (count > S.cap) {
+    S.tests = DevAlloc(sizeof(int2) * count);
+    S.fail = DevAlloc(count);
+    S.cap = count;
+  }
+  if (!S.counter.p) {
+    S.counter = DevAlloc(sizeof(unsigned long long));
+    S.revs = DevAlloc(sizeof(unsigned long long));
+  }
+  HIPC(hipMemcpyAsync(S.tests.p, tests_host, sizeof(int2) * count, hipMemcpyHostToDevice, m->stream));
+  HIPC(hipMemsetAsync(S.counter.p, 0, sizeof(unsigned long long), m->stream));
+  HIPC(hipMemsetAsync(S.revs.p, 0, sizeof(unsigned long long), m->stream));
+  const int L = m->current_level;
+  DevNet net = m->dm.net();
+  const long long bytes = wg_mem_bytes(m->dm.n, m->dm.W);
+  const int threads = 256;
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  HIPC(hipEventRecord(e0, m->stream));
+  if (bytes <= kLdsBudget) {
+    int per_cu = 0;
+    HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_singleton_tests<true>, threads, (size_t)bytes));
+    per_cu = std::max(1, per_cu);
+    const long long grid = std::min<long long>(count, (long long)per_cu * m->props.cus);
+    hipLaunchKernelGGL(k_singleton_tests<true>, dim3((unsigned)grid), dim3(threads), (size_t)bytes, m->stream, net,
+                       m->level_dom(L), m->level_size(L), S.tests.as<int2>(), count, S.fail.as<unsigned char>(),
+                       S.counter.as<unsigned long long>(), S.revs.as<unsigned long long>(), (char*)nullptr, 0ll);
+  } else {
+    const long long grid = std::min<long long>(count, 2ll * m->props.cus);
+    char* scratch = m->get_scratch(bytes * grid);
+    hipLaunchKernelGGL(k_singleton_tests<false>, dim3((unsigned)grid), dim3(threads), 0, m->stream, net,
+                       m->level_dom(L), m->level_size(L), S.tests.as<int2>(), count, S.fail.as<unsigned char>(),
+                       S.counter.as<unsigned long long>(), S.revs.as<unsigned long long>(), scratch, bytes);
+  }
+  HIPC(hipGetLastError());
+  HIPC(hipEventRecord(e1, m->stream));
+  HIPC(hipMemcpyAsync(fail_host, S.fail.p, count, hipMemcpyDeviceToHost, m->stream));
+  unsigned long long revs = 0;
+  HIPC(hipMemcpyAsync(&revs, S.revs.p, sizeof(revs), hipMemcpyDeviceToHost, m->stream));
+  m->sync();
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  st->kernel_ms += ms;
+  st->revisions += (int64_t)revs;
+  st->singleton_tests += count;
+}
+
+// One host-driven fixpoint per test on a scratch level: copy the level,
+// reduce x to {a}, seed the constraints of x, rounds with a sync each.
+void run_singleton_plain(cpim_gmodel* m, const int2* tests, long long count, unsigned char* fail_host,
+                         cpim_sac_stats* st) {
+  const int L = m->current_level;
+  if (L + 1 >= m->max_depth) fail(CPIM_ERR_RUNTIME, "no free level for singleton tests");
+  const int T = L + 1;
+  const long long t0 = now_ns();
+  std::vector<u64> word(m->dm.W);
+  const int one = 1;
+  for (long long i = 0; i < count; ++i) {
+    const int x = tests[i].x, a = tests[i].y;
+    HIPC(hipMemcpyAsync(m->level_dom(T), m->level_dom(L), sizeof(u64) * m->level_words(),
+                        hipMemcpyDeviceToDevice, m->stream));
+    HIPC(hipMemcpyAsync(m->level_size(T), m->level_size(L), sizeof(int) * m->dm.n, hipMemcpyDeviceToDevice,
+                        m->stream));
+    std::fill(word.begin(), word.end(), 0ull);
+    word[a >> 6] = 1ull << (a & 63);
+    HIPC(hipMemcpyAsync(m->level_dom(T) + (long long)x * m->dm.W, word.data(), sizeof(u64) * m->dm.W,
+                        hipMemcpyHostToDevice, m->stream));
+    HIPC(hipMemcpyAsync(m->level_size(T) + x, &one, sizeof(int), hipMemcpyHostToDevice, m->stream));
+    m->current_level = T;
+    cpim_gac_stats gs;
+    std::memset(&gs, 0, sizeof(gs));
+    try {
+      gac_grid(m, x, false, &gs);
+    } catch (...) {
+      m->current_level = L;
+      throw;
+    }
+    m->current_level = L;
+    fail_host[i] = gs.inconsistent ? 1 : 0;
+    st->revisions += gs.revisions;
+  }
+  st->kernel_ms += (now_ns() - t0) * 1e-6;
+  st->singleton_tests += count;
+}
+
+std::vector<int2> all_tests(cpim_gmodel* m, const std::vector<u64>& dom) {
+  std::vector<int2> t;
+  for (int v = 0; v < m->dm.n; ++v)
+    for (int a = 0; a < m->im.cap[v]; ++a)
+      if ((dom[(long long)v * m->dm.W + (a >> 6)] >> (a & 63)) & 1ull) t.push_back(make_int2(v, a));
+  return t;
+}
+
+std::vector<u64> read_level(cpim_gmodel* m, int level) {
+  std::vector<u64> d(m->level_words());
+  HIPC(hipMemcpyAsync(d.data(), m->level_dom(level), sizeof(u64) * d.size(), hipMemcpyDeviceToHost, m->stream));
+  m->sync();
+  return d;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpim_gmodel_singleton_tests(cpim_gmodel* m, const int32_t* tests, int64_t count, int32_t mode, uint8_t* fail_out,
+                                cpim_sac_stats* st) {
+  return guarded([&] {
+    REQUIRE(m && st && (count == 0 || (tests && fail_out)), "null argument");
+    REQUIRE(count >= 0, "negative test count");
+    DeviceGuard g(m->device);
+    std::memset(st, 0, sizeof(*st));
+    const int2* t = reinterpret_cast<const int2*>(tests);
+    for (long long i = 0; i < count; ++i) {
+      REQUIRE(t[i].x >= 0 && t[i].x < m->dm.n, "test names an unknown variable");
+      REQUIRE(t[i].y >= 0 && t[i].y < m->im.cap[t[i].x], "test value outside the domain");
+    }
+    if (mode == CPIM_SAC_BATCHED) {
+      SacScratch S;
+      run_singleton_batched(m, t, count, fail_out, st, S);
+    } else if (mode == CPIM_SAC_PLAIN) {
+      run_singleton_plain(m, t, count, fail_out, st);
+    } else {
+      fail(CPIM_ERR_INVALID_ARGUMENT, "unknown singleton-test mode");
+    }
+    st->consistent = 1;
+    st->passes = 1;
+  });
+}
+
+// SAC closure by passes: every singleton test of the current domains (one
+// batched launch), removal of the failed values, GAC from the variables that
+// lost values, until a pass removes nothing.  The closure is unique, so the
+// result equals SAC1::enforce (src/SAC1.cpp:57-103).
+int cpim_gmodel_enforce_sac(cpim_gmodel* m, int32_t mode, cpim_sac_stats* st) {
+  return guarded([&] {
+    REQUIRE(m && st, "null argument");
+    REQUIRE(mode == CPIM_SAC_BATCHED || mode == CPIM_SAC_PLAIN, "unknown SAC mode");
+    DeviceGuard g(m->device);
+    std::memset(st, 0, sizeof(*st));
+    const int L = m->current_level;
+    cpim_gac_stats gs;
+    gac(m, -1, CPIM_GAC_AUTO, &gs);
+    st->revisions += gs.revisions;
+    if (gs.inconsistent) {
+      st->consistent = 0;
+      return;
+    }
+    SacScratch S;
+    while (true) {
+      std::vector<u64> dom = read_level(m, L);
+      std::vector<int2> tests = all_tests(m, dom);
+      std::vector<unsigned char> fl(tests.size(), 0);
+      if (mode == CPIM_SAC_BATCHED) run_singleton_batched(m, tests.data(), (long long)tests.size(), fl.data(), st, S);
+      else run_singleton_plain(m, tests.data(), (long long)tests.size(), fl.data(), st);
+      ++st->passes;
+      long long removed = 0;
+      for (size_t i = 0; i < tests.size(); ++i)
+        if (fl[i]) {
+          dom[(long long)tests[i].x * m->dm.W + (tests[i].y >> 6)] &= ~(1ull << (tests[i].y & 63));
+          ++removed;
+        }
+      if (removed == 0) break;
+      st->removed += removed;
+      HIPC(hipMemcpyAsync(m->level_dom(L), dom.data(), sizeof(u64) * dom.size(), hipMemcpyHostToDevice, m->stream));
+      hipLaunchKernelGGL(k_sizes, dim3((m->dm.n + 255) / 256), dim3(256), 0, m->stream, m->level_dom(L),
+                         m->level_size(L), m->dm.n, m->dm.W);
+      HIPC(hipGetLastError());
+      m->sync();
+      gac(m, -1, CPIM_GAC_AUTO, &gs);
+      st->revisions += gs.revisions;
+      if (gs.inconsistent) {
+        st->consistent = 0;
+        return;
+      }
+    }
+    st->consistent = 1;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Device MAC
+// ---------------------------------------------------------------------------
+int cpim_gmodel_solve(cpim_gmodel* m, const cpim_solve_opts* opts, cpim_search_stats* st, int32_t* solution,
+                      cpim_node_record* trace, int32_t* trace_len) {
+  return guarded([&] {
+    REQUIRE(m && st, "null argument");
+    cpim_solve_opts o{};
+    if (opts) o = *opts;
+    REQUIRE(o.branching == CPIM_BRANCH_BINARY, "the device solver implements binary branching only");
+    REQUIRE(o.trace_capacity >= 0, "negative trace capacity");
+    REQUIRE(o.trace_capacity == 0 || trace, "trace buffer is null");
+    REQUIRE(m->current_level == 0, "the device solver starts from level 0");
+    DeviceGuard g(m->device);
+    std::memset(st, 0, sizeof(*st));
+    const int n = m->dm.n;
+    const long long t0 = now_ns();
+    DevAlloc state(sizeof(MacState)), stack(sizeof(int2) * n), sol(sizeof(int) * n);
+    DevAlloc tr(sizeof(NodeRec) * std::max(1, o.trace_capacity));
+    MacState s0{};
+    s0.status = MAC_INIT;
+    HIPC(hipMemcpyAsync(state.p, &s0, sizeof(s0), hipMemcpyHostToDevice, m->stream));
+    MacArgs A{};
+    A.state = state.as<MacState>();
+    A.trail = m->level_dom(0);
+    A.stack = stack.as<int2>();
+    A.solution = sol.as<int>();
+    A.trace = o.trace_capacity > 0 ? tr.as<NodeRec>() : nullptr;
+    A.trace_cap = o.trace_capacity;
+    A.node_limit = o.node_limit;
+    A.step_budget = 0;
+    A.deadline = 0;
+    A.root_out = nullptr;
+    DevNet net = m->dm.net();
+    const long long bytes = wg_mem_bytes(n, m->dm.W);
+    const int threads = 1024;
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    HIPC(hipEventRecord(e0, m->stream));
+    // Every launch runs a bounded number of nodes (so no launch can run for
+    // long); the search state stays on the device between launches and the
+    // host checks the time limit in between.
+    A.step_budget = kNodesPerLaunch;
+    MacState s{};
+    while (true) {
+      if (bytes <= kLdsBudget) {
+        hipLaunchKernelGGL(k_mac<true>, dim3(1), dim3(threads), (size_t)bytes, m->stream, net, A, (char*)nullptr);
+      } else {
+        char* scratch = m->get_scratch(bytes);
+        hipLaunchKernelGGL(k_mac<false>, dim3(1), dim3(threads), 0, m->stream, net, A, scratch);
+      }
+      HIPC(hipGetLastError());
+      HIPC(hipMemcpyAsync(&s, state.p, sizeof(s), hipMemcpyDeviceToHost, m->stream));
+      m->sync();
+      if (s.status != MAC_RUNNING) break;
+      if (o.time_limit_ms > 0 && (now_ns() - t0) / 1000000 > o.time_limit_ms) {
+        st->time_out = 1;
+        break;
+      }
+    }
+    HIPC(hipEventRecord(e1, m->stream));
+    HIPC(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    st->num_positive = s.npos;
+    st->num_negative = s.nneg;
+    st->num_solutions = s.status == MAC_SAT ? 1 : 0;
+    st->unsolvable = s.status == MAC_UNSAT;
+    if (s.status == MAC_LIMIT) st->time_out = 1;
+    st->gac_iterations = s.npos + s.nneg + 1;
+    st->solve_time = (now_ns() - t0) * 1e-9;
+    st->gac_time = ms * 1e-3;
+    if (solution) {
+      if (s.status == MAC_SAT) {
+        HIPC(hipMemcpyAsync(solution, sol.p, sizeof(int) * n, hipMemcpyDeviceToHost, m->stream));
+      } else {
+        for (int v = 0; v < n; ++v) solution[v] = -1;
+      }
+    }
+    const int tl = std::min(s.trace_len, o.trace_capacity);
+    if (tl > 0) {
+      std::vector<NodeRec> h(tl);
+      HIPC(hipMemcpyAsync(h.data(), tr.p, sizeof(NodeRec) * tl, hipMemcpyDeviceToHost, m->stream));
+      m->sync();
+      for (int i = 0; i < tl; ++i) {
+        trace[i].positive = h[i].positive;
+        trace[i].var = h[i].var;
+        trace[i].value = h[i].value;
+        trace[i].consistent = h[i].consistent;
+        trace[i].dom_hash = h[i].dom_hash;
+      }
+    }
+    if (trace_len) *trace_len = s.trace_len;
+    // level 0 holds the depth-0 state; its sizes must be exact again
+    hipLaunchKernelGGL(k_sizes, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->level_dom(0), m->level_size(0), n,
+                       m->dm.W);
+    HIPC(hipGetLastError());
+    m->sync();
+  });
+}
+
+}  // extern "C"
+
+#include "batch.inc"

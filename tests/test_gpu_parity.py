@@ -1,0 +1,313 @@
+"""Device engine vs the oracle (needs an MI355X).
+
+Bit-exact parity: identical fixpoint domains, identical wipe-out detection,
+identical MAC node counts, node sequences (with domain hashes) and solutions
+under dom-min / min-value ordering, on the reference's fixtures (golden
+files), on seeded random networks checked against the oracle at run time,
+and at full benchmark sizes through size-independent properties.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden_names, load_golden
+from cpim_amd import GModelAdapter, GModelOptions, GModelSolver, workload
+from cpim_amd import _native as N
+from cpim_amd.model import domains_to_sets, from_tuples
+
+pytestmark = pytest.mark.gpu
+
+GAC_MODES = {"plain": N.CPIM_GAC_PLAIN, "persistent": N.CPIM_GAC_PERSISTENT,
+             "block": N.CPIM_GAC_BLOCK}
+
+
+def popcount(dom):
+    return int(sum(bin(int(x)).count("1") for x in dom.reshape(-1)))
+
+
+def all_tests(dom, desc):
+    return np.array([(v, a) for v, vals in enumerate(domains_to_sets(dom, desc.dom_size))
+                     for a in vals], np.int32).reshape(-1, 2)
+
+
+# ---------------------------------------------------------------------------
+# Golden fixtures (the reference's own instances)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("mode", sorted(GAC_MODES))
+@pytest.mark.parametrize("name", golden_names())
+def test_golden_root_gac(gpu, name, mode):
+    desc, g = load_golden(name)
+    with GModelAdapter.Build(desc) as m:
+        before = popcount(m.ReadDomains(0))
+        st = m._gac(-1, GAC_MODES[mode])
+        assert st.inconsistent == (not bool(g["root_ok"]))
+        if g["root_ok"]:
+            dom = m.ReadDomains(0)
+            assert (dom == g["root_dom"]).all()
+            assert st.deletions == before - popcount(dom)
+            assert st.converged
+            sizes = [m.GetDomainSize(v, 0) for v in range(desc.num_vars)]
+            assert sizes == [len(s) for s in domains_to_sets(dom, desc.dom_size)]
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_golden_sac(gpu, name):
+    desc, g = load_golden(name)
+    with GModelAdapter.Build(desc) as m:
+        st = m.EnforceSAC(batched=True)
+        assert st.consistent == bool(g["sac_ok"])
+        if g["sac_ok"]:
+            assert (m.ReadDomains(0) == g["sac_dom"]).all()
+
+
+@pytest.mark.parametrize("name", ["test", "queens-4_ext", "queens-12_ext",
+                                  "composed-25-1-2-2_ext"])
+def test_golden_sac_plain_path(gpu, name):
+    desc, g = load_golden(name)
+    with GModelAdapter.Build(desc) as m:
+        st = m.EnforceSAC(batched=False)
+        assert st.consistent == bool(g["sac_ok"])
+        if g["sac_ok"]:
+            assert (m.ReadDomains(0) == g["sac_dom"]).all()
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_golden_mac(gpu, name):
+    desc, g = load_golden(name)
+    cap = len(g["mac_trace"])
+    with GModelAdapter.Build(desc) as m:
+        st, sol, trace = m.Solve(node_limit=int(g["node_limit"]), trace_capacity=cap)
+        assert (st.num_solutions, st.num_positive, st.num_negative, int(st.time_out)) == \
+            (int(g["mac_num_sol"]), int(g["mac_pos"]), int(g["mac_neg"]), int(g["mac_timeout"]))
+        assert np.array(trace, dtype=np.uint64).reshape(-1, 5).tolist() == g["mac_trace"].tolist()
+        if st.num_solutions:
+            assert list(sol) == list(g["mac_solution"])
+
+
+def test_known_answers_on_device(gpu):
+    desc, _ = load_golden("test")
+    with GModelAdapter.Build(desc) as m:
+        s = GModelSolver(m)
+        st = s.Solve()
+        assert s.GetSolution() == [0, 0, 1] and (st.num_positive, st.num_negative) == (3, 0)
+    desc, _ = load_golden("queens-4_ext")
+    with GModelAdapter.Build(desc) as m:
+        s = GModelSolver(m)
+        s.Solve()
+        assert s.GetSolution() == [1, 3, 0, 2]
+    desc, _ = load_golden("composed-25-1-2-2_ext")
+    with GModelAdapter.Build(desc) as m:
+        st = GModelSolver(m).Solve()
+        assert st.num_solutions == 0 and not st.time_out
+
+
+def test_dway_solver_reference_semantics(gpu):
+    # GModelSolver's d-way search on queens-4: 2 positive nodes (V0=0 fails,
+    # V0=1 solves), solution [1,3,0,2] (docs/GPU_PERFORMANCE_ANALYSIS.md:28,99)
+    desc, _ = load_golden("queens-4_ext")
+    with GModelAdapter.Build(desc) as m:
+        s = GModelSolver(m, branching="dway")
+        st = s.Solve()
+        assert s.GetSolution() == [1, 3, 0, 2]
+        assert (st.num_positive, st.num_negative) == (2, 1)
+        assert m.GetCurrentLevel() == 0
+
+
+# ---------------------------------------------------------------------------
+# Level / trail API: reference return conventions (src/GModel.cu:1397-1701)
+# ---------------------------------------------------------------------------
+def test_level_api_conventions(gpu):
+    desc, _ = load_golden("queens-4_ext")
+    with GModelAdapter.Build(desc) as m:
+        assert m.GetCurrentLevel() == 0 and m.max_depth == 5
+        assert m.GetMinDomainVar(0) == 0
+        assert m.GetFirstValue(1, 0) == 0 and m.GetNextValue(1, 0, 0) == 1
+        assert m.GetNextValue(1, 3, 0) == -1
+        L = m.CreateNewLevel()
+        assert L == 1 and (m.ReadDomains(1) == m.ReadDomains(0)).all()
+        assert m.AssignValue(0, 2, L)
+        assert m.GetDomainSize(0, L) == 1 and m.IsAssigned(0, L) and m.GetAssignedValue(0, L) == 2
+        assert m.GetAssignedValue(1, L) == -1
+        assert not m.RemoveValue(0, 1, L)          # absent value
+        assert m.RemoveValue(1, 3, L) and m.GetDomainSize(1, L) == 3
+        assert not m.RemoveValue(0, 2, L)          # domain becomes empty
+        st = m.EnforceGAC(False, 1)
+        assert st.inconsistent                     # var 0 is empty
+        m.BackToLevel(0)
+        assert m.GetCurrentLevel() == 0 and m.GetDomainSize(0, 0) == 4
+        with pytest.raises(RuntimeError, match="Cannot back to future level"):
+            m.BackToLevel(1)
+        with pytest.raises(RuntimeError, match="Invalid var"):
+            m.GetDomainSize(7, 0)
+        with pytest.raises(RuntimeError, match="Invalid level"):
+            m.GetDomainSize(0, 9)
+        for _ in range(4):
+            m.CreateNewLevel()
+        with pytest.raises(RuntimeError, match="Max depth reached"):
+            m.CreateNewLevel()
+        assert not m.IsFullyAssigned(0)
+
+
+def test_incremental_gac_from_assignment(gpu):
+    desc = workload.random_binary(40, 12, 160, 0.38, 31)
+    o = oracle.Oracle(desc)
+    ok, root, _ = o.root_ac()
+    assert ok
+    with GModelAdapter.Build(desc) as m:
+        m.EnforceGAC(False)
+        assert (m.ReadDomains(0) == root).all()
+        for mode in GAC_MODES.values():
+            L = m.CreateNewLevel()
+            m.AssignValue(3, int(domains_to_sets(root, desc.dom_size)[3][0]), L)
+            st = m._gac(3, mode)
+            init = root.copy()
+            a = domains_to_sets(root, desc.dom_size)[3][0]
+            init[3] = 0
+            init[3, a // 64] = np.uint64(1) << np.uint64(a % 64)
+            bok, bdom = oracle.bf_gac(desc.with_init_dom(init))
+            assert st.inconsistent == (not bok)
+            if bok:
+                assert (m.ReadDomains(L) == bdom).all()
+            m.BackToLevel(0)
+
+
+# ---------------------------------------------------------------------------
+# Seeded random networks vs the oracle (single and multi-word domains)
+# ---------------------------------------------------------------------------
+RANDOM = [  # (n, d, e, p, preremove)
+    (30, 16, 120, 0.40, 0.0), (60, 10, 400, 0.42, 0.0), (50, 64, 300, 0.25, 0.3),
+    (40, 100, 150, 0.80, 0.0), (30, 130, 60, 0.93, 0.0), (25, 256, 50, 0.97, 0.2),
+    (80, 20, 600, 0.55, 0.0),
+]
+
+
+@pytest.mark.parametrize("case", range(len(RANDOM)))
+def test_random_gac_all_modes(gpu, case):
+    n, d, e, p, pre = RANDOM[case]
+    desc = workload.random_binary(n, d, e, p, 500 + case, preremove=pre)
+    ok, dom, _ = oracle.Oracle(desc).root_ac()
+    for mode, code in GAC_MODES.items():
+        with GModelAdapter.Build(desc) as m:
+            st = m._gac(-1, code)
+            assert st.inconsistent == (not ok), mode
+            if ok:
+                assert (m.ReadDomains(0) == dom).all(), mode
+
+
+@pytest.mark.parametrize("case", range(len(RANDOM)))
+def test_random_sac_and_singletons(gpu, case):
+    n, d, e, p, pre = RANDOM[case]
+    desc = workload.random_binary(n, d, e, p, 900 + case, preremove=pre)
+    ok, dom, _ = oracle.Oracle(desc).sac1()
+    with GModelAdapter.Build(desc) as m:
+        st = m.EnforceSAC()
+        assert st.consistent == ok
+        if ok:
+            assert (m.ReadDomains(0) == dom).all()
+    o = oracle.Oracle(desc)
+    rok, root, _ = o.root_ac()
+    if rok:
+        tests = all_tests(root, desc)
+        want, _ = o.singleton_tests(tests, nthreads=4)
+        with GModelAdapter.Build(desc) as m:
+            m.EnforceGAC(False)
+            got, _ = m.SingletonTests(tests, batched=True)
+            assert (got == want).all()
+            got2, _ = m.SingletonTests(tests[:64], batched=False)
+            assert (got2 == want[:64]).all()
+
+
+@pytest.mark.parametrize("case", range(len(RANDOM)))
+def test_random_mac_traces(gpu, case):
+    n, d, e, p, pre = RANDOM[case]
+    desc = workload.random_binary(n, d, e, p, 1300 + case, preremove=pre)
+    r = oracle.Oracle(desc).mac(node_limit=3000, trace_cap=6000)
+    with GModelAdapter.Build(desc) as m:
+        st, sol, trace = m.Solve(node_limit=3000, trace_capacity=6000)
+    assert (st.num_solutions, st.num_positive, st.num_negative, st.time_out) == \
+        (r.num_sol, r.num_positive, r.num_negative, r.time_out)
+    assert trace == r.trace
+    if r.num_sol:
+        assert list(sol) == list(r.solution)
+
+
+def test_mac_across_launch_slices(gpu):
+    # more nodes than one launch runs: the device state is resumed
+    desc = workload.random_binary(23, 23, 253, 0.247, 4)
+    r = oracle.Oracle(desc).mac(node_limit=40000)
+    with GModelAdapter.Build(desc) as m:
+        st, sol, _ = m.Solve(node_limit=40000)
+    assert (st.num_solutions, st.num_positive, st.num_negative) == \
+        (r.num_sol, r.num_positive, r.num_negative)
+
+
+# ---------------------------------------------------------------------------
+# Positive tables (arity 3) when skip_non_binary is off
+# ---------------------------------------------------------------------------
+def ternary(seed):
+    base = workload.random_binary(14, 6, 25, 0.3, seed)
+    scopes = [list(s) for s in base.scopes()]
+    lists = list(base.tuple_lists())
+    rng = np.random.default_rng(seed)
+    for k in range(5):
+        scopes.append(sorted(rng.choice(14, 3, replace=False).tolist()))
+        lists.append(workload.random_table(seed * 17 + k, 3, 6, 60))
+    return from_tuples(base.dom_size, scopes, lists)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_tables_gac_and_mac(gpu, seed):
+    desc = ternary(seed)
+    opts = GModelOptions(skip_non_binary=False)
+    ok, dom, _ = oracle.Oracle(desc, oracle.AC3).root_ac()
+    for mode, code in GAC_MODES.items():
+        with GModelAdapter.Build(desc, opts) as m:
+            st = m._gac(-1, code)
+            assert st.inconsistent == (not ok), mode
+            if ok:
+                assert (m.ReadDomains(0) == dom).all(), mode
+    r = oracle.Oracle(desc, oracle.AC3).mac(node_limit=2000, trace_cap=4000)
+    with GModelAdapter.Build(desc, opts) as m:
+        st, sol, trace = m.Solve(node_limit=2000, trace_capacity=4000)
+    assert (st.num_solutions, st.num_positive, st.num_negative) == \
+        (r.num_sol, r.num_positive, r.num_negative)
+    assert trace == r.trace
+
+
+# ---------------------------------------------------------------------------
+# Full benchmark sizes: properties + sampled oracle checks
+# ---------------------------------------------------------------------------
+def test_single_large_full_size(gpu):
+    desc = workload.single_large()
+    ok, dom, _ = oracle.Oracle(desc).root_ac()
+    results = {}
+    for mode, code in GAC_MODES.items():
+        with GModelAdapter.Build(desc) as m:
+            st = m._gac(-1, code)
+            assert st.inconsistent == (not ok)
+            results[mode] = m.ReadDomains(0)
+            assert (results[mode] == dom).all(), mode
+            # idempotence: a second fixpoint from every constraint changes nothing
+            st2 = m._gac(-1, code)
+            assert st2.deletions == 0 and (m.ReadDomains(0) == dom).all()
+
+
+def test_batched_sac_full_size_sample(gpu):
+    desc = workload.batched_sac()
+    o = oracle.Oracle(desc)
+    ok, root, _ = o.root_ac()
+    assert ok
+    tests = all_tests(root, desc)
+    rng = np.random.default_rng(3)
+    sample = tests[np.sort(rng.choice(len(tests), 96, replace=False))]
+    want, _ = o.singleton_tests(sample, nthreads=8)
+    with GModelAdapter.Build(desc) as m:
+        st = m.EnforceGAC_Auto()
+        assert not st.inconsistent and (m.ReadDomains(0) == root).all()
+        got, sst = m.SingletonTests(sample)
+        assert (got == want).all()
+        # every singleton test of the full pass, in one launch
+        allf, sst = m.SingletonTests(tests)
+        assert sst.singleton_tests == len(tests)
+        assert (allf[np.searchsorted(tests[:, 0] * 256 + tests[:, 1],
+                                     sample[:, 0] * 256 + sample[:, 1])] == want).all()

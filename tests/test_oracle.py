@@ -1,0 +1,243 @@
+"""Pins of the CPU oracle (CPU only).
+
+The oracle restates the reference's AC3bit / AC3 / SAC1 / MAC; these tests
+tie it to what the reference itself holds (known answers on its fixtures),
+to definitions (brute-force GAC / SAC closures, exhaustive enumeration, a
+second MAC driver whose propagation is the definitional closure), to closed
+forms, and show that each planted mistake is caught by at least one pin.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden_names, load_golden
+from cpim_amd import workload
+from cpim_amd.model import ModelDesc, dense_from_tuples, domains_to_sets, from_tuples
+
+
+def check_solution(desc, sol):
+    """Every constraint is satisfied by sol (definitional)."""
+    for scope, tup in zip(desc.scopes(), desc.tuple_lists()):
+        t = tuple(int(sol[v]) for v in scope)
+        if not any(tuple(row) == t for row in tup):
+            return False
+    return True
+
+
+# ---------------------------------------------------------------------------
+# Known answers held by the reference (docs/GPU_PERFORMANCE_ANALYSIS.md:28-30)
+# and hand-derived ones (samples/bench/test.xml)
+# ---------------------------------------------------------------------------
+def test_known_answer_test_xml():
+    desc, g = load_golden("test")
+    ok, dom, _ = oracle.Oracle(desc).root_ac()
+    assert ok
+    # V0 = V1, V1 < V2 over {0,1,2}: root GAC gives {0,1}, {0,1}, {1,2}
+    assert domains_to_sets(dom, desc.dom_size) == [[0, 1], [0, 1], [1, 2]]
+    r = oracle.Oracle(desc).mac()
+    assert r.num_sol == 1 and list(r.solution) == [0, 0, 1]
+    assert (r.num_positive, r.num_negative) == (3, 0)
+
+
+def test_known_answer_queens4():
+    desc, _ = load_golden("queens-4_ext")
+    r = oracle.Oracle(desc).mac()
+    assert r.num_sol == 1 and list(r.solution) == [1, 3, 0, 2]
+
+
+def test_known_answer_composed_unsat():
+    desc, _ = load_golden("composed-25-1-2-2_ext")
+    r = oracle.Oracle(desc).mac()
+    assert r.num_sol == 0 and not r.time_out
+
+
+# ---------------------------------------------------------------------------
+# Golden fixtures: stored expectations equal the oracle and brute force
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", golden_names())
+def test_golden_matches_oracle_and_bruteforce(name):
+    desc, g = load_golden(name)
+    ok, dom, _ = oracle.Oracle(desc).root_ac()
+    assert ok == bool(g["root_ok"])
+    if ok:
+        assert (dom == g["root_dom"]).all()
+    bok, bdom = oracle.bf_gac(desc)
+    assert bok == ok and (not ok or (bdom == dom).all())
+    sok, sdom, _ = oracle.Oracle(desc).sac1()
+    assert sok == bool(g["sac_ok"])
+    if sok:
+        assert (sdom == g["sac_dom"]).all()
+    r = oracle.Oracle(desc).mac(node_limit=int(g["node_limit"]), trace_cap=len(g["mac_trace"]))
+    assert (r.num_sol, r.num_positive, r.num_negative, int(r.time_out)) == \
+        (int(g["mac_num_sol"]), int(g["mac_pos"]), int(g["mac_neg"]), int(g["mac_timeout"]))
+    assert np.array(r.trace, dtype=np.uint64).reshape(-1, 5).tolist() == g["mac_trace"].tolist()
+    if r.num_sol:
+        assert check_solution(desc, r.solution)
+
+
+# ---------------------------------------------------------------------------
+# Seeded random networks: restatement == definitions
+# ---------------------------------------------------------------------------
+SMALL = [  # (n, d, e, p)
+    (8, 4, 14, 0.45), (10, 5, 20, 0.5), (12, 6, 30, 0.45), (15, 8, 40, 0.4),
+    (20, 10, 60, 0.35), (12, 70, 20, 0.85), (10, 130, 12, 0.93),
+]
+
+
+@pytest.mark.parametrize("case", range(len(SMALL)))
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_gac_sac_mac_vs_bruteforce(case, seed):
+    n, d, e, p = SMALL[case]
+    desc = workload.random_binary(n, d, e, p, 1000 * case + seed)
+    ok, dom, _ = oracle.Oracle(desc).root_ac()
+    bok, bdom = oracle.bf_gac(desc)
+    assert ok == bok and (not ok or (dom == bdom).all())
+    sok, sdom, _ = oracle.Oracle(desc).sac1()
+    bsok, bsdom = oracle.bf_sac(desc)
+    assert sok == bsok and (not sok or (sdom == bsdom).all())
+    r = oracle.Oracle(desc).mac(node_limit=2000, trace_cap=4000)
+    b = oracle.bf_mac(desc, node_limit=2000, trace_cap=4000)
+    assert (r.num_sol, r.num_positive, r.num_negative, r.time_out) == \
+        (b.num_sol, b.num_positive, b.num_negative, b.time_out)
+    assert r.trace == b.trace
+    if r.num_sol:
+        assert check_solution(desc, r.solution)
+    if d ** n <= 10 ** 7 and not r.time_out:
+        count, _ = oracle.bf_count_solutions(desc, limit=1)
+        assert (count > 0) == (r.num_sol > 0)
+
+
+def test_pre_removed_domains_vs_bruteforce():
+    desc = workload.random_binary(30, 64, 120, 0.25, 77, preremove=0.3)
+    ok, dom, _ = oracle.Oracle(desc).root_ac()
+    bok, bdom = oracle.bf_gac(desc)
+    assert ok == bok and (not ok or (dom == bdom).all())
+
+
+def ternary_model(seed, n=9, d=5, e=10, n_tab=4, ntup=40, p=0.3):
+    base = workload.random_binary(n, d, e, p, seed)
+    scopes = [list(s) for s in base.scopes()]
+    lists = list(base.tuple_lists())
+    rng = np.random.default_rng(seed)
+    for k in range(n_tab):
+        scopes.append(sorted(rng.choice(n, 3, replace=False).tolist()))
+        lists.append(workload.random_table(seed * 31 + k, 3, d, ntup))
+    return from_tuples(base.dom_size, scopes, lists)
+
+
+@pytest.mark.parametrize("seed", [5, 6, 7, 8])
+def test_ternary_tables_generic_ac3_vs_bruteforce(seed):
+    desc = ternary_model(seed)
+    o = oracle.Oracle(desc, oracle.AC3)
+    ok, dom, _ = o.root_ac()
+    bok, bdom = oracle.bf_gac(desc)
+    assert ok == bok and (not ok or (dom == bdom).all())
+    r = oracle.Oracle(desc, oracle.AC3).mac(node_limit=3000, trace_cap=4000)
+    b = oracle.bf_mac(desc, node_limit=3000, trace_cap=4000)
+    assert (r.num_sol, r.num_positive, r.num_negative) == (b.num_sol, b.num_positive, b.num_negative)
+    assert r.trace == b.trace
+    if r.num_sol:
+        assert check_solution(desc, r.solution)
+
+
+def test_dense_and_tuple_encodings_agree():
+    desc = workload.random_binary(12, 9, 30, 0.4, 4242)
+    tup = from_tuples(desc.dom_size, desc.scopes(), desc.tuple_lists())
+    for d in (desc, tup, dense_from_tuples(tup)):
+        ok, dom, _ = oracle.Oracle(d).root_ac()
+        ok3, dom3, _ = oracle.Oracle(d, oracle.AC3).root_ac()
+        assert ok == ok3 and (dom == dom3).all()
+    a = oracle.Oracle(desc).mac(trace_cap=1000)
+    b = oracle.Oracle(tup, oracle.AC3).mac(trace_cap=1000)
+    assert (a.num_positive, a.num_negative, a.trace) == (b.num_positive, b.num_negative, b.trace)
+
+
+# ---------------------------------------------------------------------------
+# Closed forms
+# ---------------------------------------------------------------------------
+def universal_model(n, d):
+    scopes = [(i, i + 1) for i in range(n - 1)]
+    tuples = [np.array(list(itertools.product(range(d), range(d))), np.int32) for _ in scopes]
+    return dense_from_tuples(from_tuples([d] * n, scopes, tuples))
+
+
+def test_universal_relations_closed_form():
+    # every tuple allowed: nothing is pruned, MAC assigns 0 to every variable
+    # with n positive and no negative decision
+    desc = universal_model(7, 5)
+    ok, dom, _ = oracle.Oracle(desc).root_ac()
+    assert ok and (dom == desc.full_domains()).all()
+    r = oracle.Oracle(desc).mac()
+    assert list(r.solution) == [0] * 7 and (r.num_positive, r.num_negative) == (7, 0)
+
+
+def test_equality_chain_closed_form():
+    # x_i = x_{i+1} with x_0 in {2, 3} (pre-removed): every domain becomes {2, 3}
+    n, d = 6, 5
+    scopes = [(i, i + 1) for i in range(n - 1)]
+    tuples = [np.array([(a, a) for a in range(d)], np.int32) for _ in scopes]
+    desc = dense_from_tuples(from_tuples([d] * n, scopes, tuples))
+    init = desc.full_domains()
+    init[0, 0] = np.uint64(0b01100)
+    desc = desc.with_init_dom(init)
+    ok, dom, _ = oracle.Oracle(desc).root_ac()
+    assert ok and domains_to_sets(dom, desc.dom_size) == [[2, 3]] * n
+
+
+def test_empty_relation_wipes_out():
+    desc = dense_from_tuples(from_tuples([3, 3, 3], [(0, 1), (1, 2)],
+                                         [np.array([[0, 0], [1, 1]], np.int32),
+                                          np.zeros((0, 2), np.int32)]))
+    ok, _, _ = oracle.Oracle(desc).root_ac()
+    assert not ok
+    assert oracle.Oracle(desc).mac().num_sol == 0
+
+
+# ---------------------------------------------------------------------------
+# Singleton tests: multi-threaded == single-threaded == definition
+# ---------------------------------------------------------------------------
+def test_singleton_tests_threads_agree():
+    desc = workload.random_binary(25, 12, 90, 0.42, 99)
+    o = oracle.Oracle(desc)
+    ok, dom, _ = o.root_ac()
+    assert ok
+    tests = np.array([(v, a) for v, vals in enumerate(domains_to_sets(dom, desc.dom_size))
+                      for a in vals], np.int32)
+    f1, _ = o.singleton_tests(tests, nthreads=1)
+    f4, _ = o.singleton_tests(tests, nthreads=4)
+    assert (f1 == f4).all()
+    for (x, a), f in zip(tests, f1):
+        init = dom.copy()
+        init[x] = 0
+        init[x, a // 64] = np.uint64(1) << np.uint64(a % 64)
+        bok, _ = oracle.bf_gac(desc.with_init_dom(init))
+        assert bool(f) == (not bok)
+
+
+# ---------------------------------------------------------------------------
+# Planted mistakes: each is caught by a pin above
+# ---------------------------------------------------------------------------
+def pins_hold(desc, node_limit=2000):
+    ok, dom, _ = oracle.Oracle(desc).root_ac()
+    bok, bdom = oracle.bf_gac(desc)
+    if ok != bok or (ok and not (dom == bdom).all()):
+        return False
+    sok, sdom, _ = oracle.Oracle(desc).sac1()
+    bsok, bsdom = oracle.bf_sac(desc)
+    if sok != bsok or (sok and not (sdom == bsdom).all()):
+        return False
+    r = oracle.Oracle(desc).mac(node_limit=node_limit, trace_cap=5000)
+    b = oracle.bf_mac(desc, node_limit=node_limit, trace_cap=5000)
+    return r.trace == b.trace and (r.num_positive, r.num_negative) == (b.num_positive, b.num_negative)
+
+
+@pytest.mark.parametrize("fault", [1, 2, 3, 4])
+def test_planted_faults_are_caught(fault):
+    cases = [workload.random_binary(12, 70, 20, 0.85, 3), workload.random_binary(15, 8, 40, 0.4, 2),
+             workload.random_binary(20, 10, 60, 0.35, 5), workload.random_binary(10, 130, 12, 0.93, 1)]
+    assert all(pins_hold(c) for c in cases)
+    with oracle.fault(fault):
+        caught = not all(pins_hold(c) for c in cases)
+    assert caught, f"planted fault {fault} went unnoticed"
