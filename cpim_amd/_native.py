@@ -26,6 +26,7 @@ CPIM_GAC_PLAIN = 0
 CPIM_GAC_PERSISTENT = 1
 CPIM_GAC_BLOCK = 2
 CPIM_GAC_AUTO = 3
+CPIM_GAC_ASYNC = 4
 
 CPIM_SAC_BATCHED = 0
 CPIM_SAC_PLAIN = 1

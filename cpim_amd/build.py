@@ -59,9 +59,24 @@ def build_workload(verbose: bool = False) -> str:
     return WORKLOAD_LIB
 
 
+SOLVE_BIN = os.path.join(HERE, "cpim_solve")
+
+
+def build_cli(verbose: bool = False) -> str:
+    """cpim_solve: C++ driver over the facade, linked to both libraries."""
+    tmp = SOLVE_BIN + ".tmp"
+    cmd = ["g++", "-O2", "-march=x86-64-v2", "-std=c++17", "-Wall", "-o", tmp,
+           os.path.join(CSRC, "cpim_solve.cpp"), "-I", os.path.join(REPO, "include"),
+           "-L", HERE, "-lcpim_gpu", "-lcpim_workload", "-Wl,-rpath,$ORIGIN"]
+    _run(cmd, verbose)
+    os.replace(tmp, SOLVE_BIN)
+    return SOLVE_BIN
+
+
 def build_all(verbose: bool = False) -> None:
     build_workload(verbose)
     build_gpu(verbose)
+    build_cli(verbose)
 
 
 if __name__ == "__main__":

@@ -54,7 +54,18 @@ __device__ __forceinline__ u64 readlane64(u64 v, int lane) {
   return ((u64)hi << 32) | lo;
 }
 
+// Wave-uniform copies of values read from LDS.  Every value that decides a
+// branch or loop exit around a __syncthreads() goes through these, so the
+// compiler emits scalar branches: a value it believes divergent turns such a
+// loop into an exec-mask loop, which can keep some lanes in the loop while the
+// lane that must publish the next work item waits outside it.
 __device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ long long uniform_ll(long long v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)v & 0xffffffffull));
+  const unsigned hi = __builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
 
 __device__ __forceinline__ int mbcnt(u64 mask) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
@@ -69,6 +80,18 @@ __device__ __forceinline__ int wave_inclusive_scan(int v) {
     if (lane >= d) v += o;
   }
   return v;
+}
+
+// Domain word load.  kCoherent: agent-scope atomic load, which observes every
+// update other workgroups (on any XCD) have completed; used by the
+// asynchronous fixpoint, which has no barrier between rounds.
+template <bool kCoherent>
+__device__ __forceinline__ u64 ld_dom(const u64* p) {
+  if constexpr (kCoherent) {
+    return __hip_atomic_load(const_cast<u64*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    return *p;
+  }
 }
 
 __device__ __forceinline__ u64 mix64(u64 z) {
@@ -91,6 +114,7 @@ __device__ __forceinline__ u64 mix64(u64 z) {
 // Returns the number of values removed (wave-uniform); *wiped is set when
 // this wave emptied D(u).
 // ---------------------------------------------------------------------------
+template <bool kCoherent = false>
 __device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* size, int c, int u,
                                              int t, int pos_t, bool* wiped) {
   const int W = net.W;
@@ -99,8 +123,8 @@ __device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* s
   const u64* base = net.sup + net.con[c].off;
   const u64* plane_u = base + (long long)(1 - pos_t) * W * dpad;
   const u64* plane_t = base + (long long)pos_t * W * dpad;
-  const u64 dt_l = lane < W ? dom[(long long)t * W + lane] : 0ull;
-  const u64 du_l = lane < W ? dom[(long long)u * W + lane] : 0ull;
+  const u64 dt_l = lane < W ? ld_dom<kCoherent>(&dom[(long long)t * W + lane]) : 0ull;
+  const u64 du_l = lane < W ? ld_dom<kCoherent>(&dom[(long long)u * W + lane]) : 0ull;
   int cnt_t = 0;
   int first_w = -1;
   for (int w = 0; w < W; ++w) {
@@ -160,7 +184,7 @@ __device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* s
 // emptied a domain.  on_removed(var, count) is called (by the whole wave)
 // for every variable that lost values.  Requires tw <= 64 (4096 tuples).
 // ---------------------------------------------------------------------------
-template <typename OnRemoved>
+template <typename OnRemoved, bool kCoherent = false>
 __device__ __forceinline__ void revise_table(const DevNet& net, u64* dom, int* size, int c,
                                              int skip_var, const unsigned* asg,
                                              OnRemoved on_removed, bool* wiped) {
@@ -177,7 +201,7 @@ __device__ __forceinline__ void revise_table(const DevNet& net, u64* dom, int* s
     const int v = mp->scope[p];
     u64 acc = 0ull;
     for (int j = 0; j < W; ++j) {
-      u64 word = dom[(long long)v * W + j];
+      u64 word = ld_dom<kCoherent>(&dom[(long long)v * W + j]);
       word = readlane64(word, 0);
       while (word) {
         const int bit = (int)__builtin_ctzll(word);
@@ -193,7 +217,7 @@ __device__ __forceinline__ void revise_table(const DevNet& net, u64* dom, int* s
     const int y = mp->scope[p];
     if (y == skip_var) continue;
     if (asg && ((asg[y >> 5] >> (y & 31)) & 1u)) continue;
-    const u64 du_l = lane < W ? dom[(long long)y * W + lane] : 0ull;
+    const u64 du_l = lane < W ? ld_dom<kCoherent>(&dom[(long long)y * W + lane]) : 0ull;
     u64 mask_l = 0ull;
     for (int j = 0; j < W; ++j) {
       const u64 duj = readlane64(du_l, j);

@@ -101,6 +101,10 @@ DevProps props_of(int dev) {
 // several workgroups share a CU.
 constexpr long long kLdsBudget = 60 * 1024;
 
+// Device wall-clock budget of one asynchronous fixpoint launch (ms); a
+// launch that exceeds it returns an error instead of running on.
+constexpr long long kAsyncBudgetMs = 20000;
+
 // MAC nodes one launch may run before it returns to the host.
 constexpr long long kNodesPerLaunch = 1 << 14;
 
@@ -391,6 +395,7 @@ struct cpim_gmodel {
   DevAlloc front;  // 3 x nfw
   DevAlloc ctl;    // GacCtl
   DevAlloc block_out;
+  DevAlloc async_ctl;
   DevAlloc scratch;  // workgroup-engine scratch when a copy does not fit in LDS
   long long scratch_bytes = 0;
   int nfw = 0;
@@ -582,6 +587,39 @@ void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats*
   st->revisions = (int64_t)res.revisions;
 }
 
+void gac_async(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
+  const int L = m->current_level;
+  u64* dom = m->level_dom(L);
+  int* size = m->level_size(L);
+  unsigned* F = m->front.as<unsigned>();
+  const int nfw = m->nfw;
+  if (!m->async_ctl.p) m->async_ctl = DevAlloc(sizeof(AsyncCtl));
+  AsyncCtl* ctl = m->async_ctl.as<AsyncCtl>();
+  AsyncCtl init{};
+  HIPC(hipMemcpyAsync(ctl, &init, sizeof(init), hipMemcpyHostToDevice, m->stream));
+  HIPC(hipMemsetAsync(F, 0, sizeof(unsigned) * nfw, m->stream));
+  hipLaunchKernelGGL(k_seed_async, dim3(assigned_var < 0 ? (nfw + 255) / 256 : 1), dim3(256), 0, m->stream, F, nfw,
+                     m->dm.num_active, m->dm.adj_off.as<int>(), m->dm.adj.as<int4>(), assigned_var, ctl);
+  HIPC(hipGetLastError());
+  const int threads = 256;
+  const int waves = std::max(1, std::min(nfw, 4 * m->props.cus));
+  const int blocks = (waves + threads / 64 - 1) / (threads / 64);
+  // time budget of one launch: kAsyncBudgetMs of device wall clock
+  const unsigned long long max_ticks = (unsigned long long)std::max<long long>(m->props.wall_khz, 1) * kAsyncBudgetMs;
+  DevNet net = m->dm.net();
+  hipLaunchKernelGGL(k_gac_async, dim3(blocks), dim3(threads), 0, m->stream, net, dom, size, F, nfw, ctl, max_ticks);
+  HIPC(hipGetLastError());
+  AsyncCtl res{};
+  HIPC(hipMemcpyAsync(&res, ctl, sizeof(res), hipMemcpyDeviceToHost, m->stream));
+  m->sync();
+  if (res.err) fail(CPIM_ERR_RUNTIME, "asynchronous fixpoint exceeded its time budget");
+  st->iterations = 1;
+  st->deletions = (int32_t)res.deletions;
+  st->inconsistent = res.wipe != 0;
+  st->converged = !res.wipe && res.pending == 0;
+  st->revisions = (int64_t)res.revisions;
+}
+
 void gac_block(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
   const int L = m->current_level;
   DevNet net = m->dm.net();
@@ -612,13 +650,15 @@ int auto_gac_mode(const cpim_gmodel* m) {
   // grid barrier).  Large ones: the persistent grid when cooperative launch
   // is available.
   if (fits_lds(m) && m->dm.num_active <= 4096) return CPIM_GAC_BLOCK;
-  return m->props.coop ? CPIM_GAC_PERSISTENT : CPIM_GAC_PLAIN;
+  return CPIM_GAC_ASYNC;
 }
 
 void gac(cpim_gmodel* m, int assigned_var, int mode, cpim_gac_stats* st) {
   std::memset(st, 0, sizeof(*st));
   if (mode == CPIM_GAC_AUTO) mode = auto_gac_mode(m);
-  if (mode == CPIM_GAC_PERSISTENT && !m->props.coop) mode = CPIM_GAC_PLAIN;  // reference fallback
+  if (mode == CPIM_GAC_PERSISTENT)
+    fail(CPIM_ERR_UNSUPPORTED,
+         "persistent GAC is switched off: its first GPU run went silent and is under investigation");
   if (m->dm.num_active == 0) {
     st->converged = 1;
     return;
@@ -627,6 +667,7 @@ void gac(cpim_gmodel* m, int assigned_var, int mode, cpim_gac_stats* st) {
     case CPIM_GAC_PLAIN: gac_grid(m, assigned_var, false, st); break;
     case CPIM_GAC_PERSISTENT: gac_grid(m, assigned_var, true, st); break;
     case CPIM_GAC_BLOCK: gac_block(m, assigned_var, st); break;
+    case CPIM_GAC_ASYNC: gac_async(m, assigned_var, st); break;
     default: fail(CPIM_ERR_INVALID_ARGUMENT, "unknown GAC mode " + std::to_string(mode));
   }
 }
@@ -889,9 +930,14 @@ int cpim_gmodel_write_domains(cpim_gmodel* m, int32_t level, const uint64_t* in)
 namespace {
 
 struct SacScratch {
-  DevAlloc tests, fail, counter, revs;
+  DevAlloc tests, fail, revs;
   long long cap = 0;
 };
+
+// Launch k_singleton_tests over `count` tests: one workgroup per test.  With
+// domains in LDS the whole list is one launch; with global scratch the list
+// is cut into launches of kScratchBlocks workgroups.
+constexpr long long kScratchBlocks = 2048;
 
 void run_singleton_batched(cpim_gmodel* m, const int2* tests_host, long long count, unsigned char* fail_host,
                            cpim_sac_stats* st, SacScratch& S) {
@@ -901,12 +947,8 @@ void run_singleton_batched(cpim_gmodel* m, const int2* tests_host, long long cou
     S.fail = DevAlloc(count);
     S.cap = count;
   }
-  if (!S.counter.p) {
-    S.counter = DevAlloc(sizeof(unsigned long long));
-    S.revs = DevAlloc(sizeof(unsigned long long));
-  }
+  if (!S.revs.p) S.revs = DevAlloc(sizeof(unsigned long long));
   HIPC(hipMemcpyAsync(S.tests.p, tests_host, sizeof(int2) * count, hipMemcpyHostToDevice, m->stream));
-  HIPC(hipMemsetAsync(S.counter.p, 0, sizeof(unsigned long long), m->stream));
   HIPC(hipMemsetAsync(S.revs.p, 0, sizeof(unsigned long long), m->stream));
   const int L = m->current_level;
   DevNet net = m->dm.net();
@@ -917,21 +959,23 @@ void run_singleton_batched(cpim_gmodel* m, const int2* tests_host, long long cou
   HIPC(hipEventCreate(&e1));
   HIPC(hipEventRecord(e0, m->stream));
   if (bytes <= kLdsBudget) {
-    int per_cu = 0;
-    HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_singleton_tests<true>, threads, (size_t)bytes));
-    per_cu = std::max(1, per_cu);
-    const long long grid = std::min<long long>(count, (long long)per_cu * m->props.cus);
-    hipLaunchKernelGGL(k_singleton_tests<true>, dim3((unsigned)grid), dim3(threads), (size_t)bytes, m->stream, net,
-                       m->level_dom(L), m->level_size(L), S.tests.as<int2>(), count, S.fail.as<unsigned char>(),
-                       S.counter.as<unsigned long long>(), S.revs.as<unsigned long long>(), (char*)nullptr, 0ll);
+    for (long long first = 0; first < count; first += INT_MAX) {
+      const long long blocks = std::min<long long>(count - first, INT_MAX);
+      hipLaunchKernelGGL(k_singleton_tests<true>, dim3((unsigned)blocks), dim3(threads), (size_t)bytes, m->stream,
+                         net, m->level_dom(L), m->level_size(L), S.tests.as<int2>(), count, first,
+                         S.fail.as<unsigned char>(), S.revs.as<unsigned long long>(), (char*)nullptr, 0ll);
+      HIPC(hipGetLastError());
+    }
   } else {
-    const long long grid = std::min<long long>(count, 2ll * m->props.cus);
-    char* scratch = m->get_scratch(bytes * grid);
-    hipLaunchKernelGGL(k_singleton_tests<false>, dim3((unsigned)grid), dim3(threads), 0, m->stream, net,
-                       m->level_dom(L), m->level_size(L), S.tests.as<int2>(), count, S.fail.as<unsigned char>(),
-                       S.counter.as<unsigned long long>(), S.revs.as<unsigned long long>(), scratch, bytes);
+    char* scratch = m->get_scratch(bytes * kScratchBlocks);
+    for (long long first = 0; first < count; first += kScratchBlocks) {
+      const long long blocks = std::min<long long>(count - first, kScratchBlocks);
+      hipLaunchKernelGGL(k_singleton_tests<false>, dim3((unsigned)blocks), dim3(threads), 0, m->stream, net,
+                         m->level_dom(L), m->level_size(L), S.tests.as<int2>(), count, first,
+                         S.fail.as<unsigned char>(), S.revs.as<unsigned long long>(), scratch, bytes);
+      HIPC(hipGetLastError());
+    }
   }
-  HIPC(hipGetLastError());
   HIPC(hipEventRecord(e1, m->stream));
   HIPC(hipMemcpyAsync(fail_host, S.fail.p, count, hipMemcpyDeviceToHost, m->stream));
   unsigned long long revs = 0;
@@ -1062,11 +1106,23 @@ int cpim_gmodel_enforce_sac(cpim_gmodel* m, int32_t mode, cpim_sac_stats* st) {
         }
       if (removed == 0) break;
       st->removed += removed;
+      // a variable whose every value failed its singleton test has an empty
+      // domain: that is a wipe-out even if no constraint reaches it
+      bool empty = false;
+      for (int v = 0; v < m->dm.n && !empty; ++v) {
+        u64 any = 0;
+        for (int w = 0; w < m->dm.W; ++w) any |= dom[(long long)v * m->dm.W + w];
+        empty = any == 0;
+      }
       HIPC(hipMemcpyAsync(m->level_dom(L), dom.data(), sizeof(u64) * dom.size(), hipMemcpyHostToDevice, m->stream));
       hipLaunchKernelGGL(k_sizes, dim3((m->dm.n + 255) / 256), dim3(256), 0, m->stream, m->level_dom(L),
                          m->level_size(L), m->dm.n, m->dm.W);
       HIPC(hipGetLastError());
       m->sync();
+      if (empty) {
+        st->consistent = 0;
+        return;
+      }
       gac(m, -1, CPIM_GAC_AUTO, &gs);
       st->revisions += gs.revisions;
       if (gs.inconsistent) {

@@ -195,6 +195,153 @@ __global__ void k_seed_var(unsigned* F, const int* adj_off, const int4* adj, int
   }
 }
 
+// ===========================================================================
+// Asynchronous fixpoint in one launch, without any grid barrier.
+//
+// The frontier is a bitmap of constraints.  A wave claims a whole frontier
+// word with an atomic exchange, revises the claimed constraints and pushes
+// the constraints of every variable that lost values.  `pending` counts the
+// queued plus the claimed-but-unfinished constraints; it is raised before a
+// push sets its bits (and lowered again for bits that were already set) and
+// lowered only after a claimed constraint is finished, so it never drops below
+// the outstanding work and reaches 0 exactly when the fixpoint is reached.
+// No wave waits for another: an idle wave keeps scanning until pending is 0,
+// the wipe-out flag is set, or its time budget runs out (then it sets `err`
+// and leaves), so every wave exits and co-residency is not required.
+// Domain and frontier words are read with agent-scope atomic loads, so a
+// revision always sees removals completed by other XCDs before the push that
+// queued it.
+// ===========================================================================
+struct AsyncCtl {
+  unsigned long long pending;
+  int wipe;
+  int err;
+  unsigned long long deletions;
+  unsigned long long revisions;
+};
+
+__device__ __forceinline__ void push_var_async(const DevNet& net, int v, unsigned* F, AsyncCtl* ctl) {
+  const int lane = lane_id();
+  const int b0 = net.adj_off[v], b1 = net.adj_off[v + 1];
+  for (int i0 = b0; i0 < b1; i0 += 64) {
+    const int k = min(64, b1 - i0);
+    if (lane == 0) atomicAdd(&ctl->pending, (unsigned long long)k);
+    bool already = false;
+    if (lane < k) {
+      const int c = net.adj[i0 + lane].x;
+      const unsigned bit = 1u << (c & 31);
+      already = (atomicOr(&F[c >> 5], bit) & bit) != 0u;
+    }
+    const int dup = __popcll(__ballot(already));
+    if (dup && lane == 0) atomicSub(&ctl->pending, (unsigned long long)dup);
+  }
+}
+
+__device__ __forceinline__ void async_process_constraint(const DevNet& net, u64* dom, int* size, int c,
+                                                         unsigned* F, AsyncCtl* ctl,
+                                                         unsigned long long& revs,
+                                                         unsigned long long& dels) {
+  const ConMeta* mp = net.con + c;
+  bool wiped = false;
+  if (mp->kind == 0) {
+    const int x = mp->scope[0], y = mp->scope[1];
+    const int rx = revise_binary<true>(net, dom, size, c, x, y, 1, &wiped);
+    ++revs;
+    if (wiped && lane_id() == 0) atomicExch(&ctl->wipe, 1);
+    if (rx) {
+      dels += rx;
+      push_var_async(net, x, F, ctl);
+    }
+    const int ry = revise_binary<true>(net, dom, size, c, y, x, 0, &wiped);
+    ++revs;
+    if (wiped && lane_id() == 0) atomicExch(&ctl->wipe, 1);
+    if (ry) {
+      dels += ry;
+      push_var_async(net, y, F, ctl);
+    }
+  } else {
+    auto on_removed = [&](int v, int k) {
+      dels += k;
+      push_var_async(net, v, F, ctl);
+    };
+    revise_table<decltype(on_removed), true>(net, dom, size, c, -1, nullptr, on_removed, &wiped);
+    ++revs;
+    if (wiped && lane_id() == 0) atomicExch(&ctl->wipe, 1);
+  }
+}
+
+__global__ void k_gac_async(DevNet net, u64* dom, int* size, unsigned* F, int nfw, AsyncCtl* ctl,
+                            unsigned long long max_ticks) {
+  const int lane = lane_id();
+  const long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const unsigned long long t0 = wall_clock64();
+  const int nchunks = (nfw + 63) / 64;
+  int chunk = (int)(gw % nchunks);
+  int seen = 0;  // chunks scanned since the last claim
+  unsigned long long revs = 0, dels = 0;
+  while (true) {
+    if (__hip_atomic_load(&ctl->wipe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+        __hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      break;
+    const int wi = chunk * 64 + lane;
+    unsigned bits = 0u;
+    if (wi < nfw) bits = __hip_atomic_load(&F[wi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const u64 nz = __ballot(bits != 0u);
+    if (nz) {
+      const int l = (int)__builtin_ctzll(nz);
+      unsigned claimed = 0u;
+      if (lane == 0) claimed = atomicExch(&F[chunk * 64 + l], 0u);
+      claimed = __shfl(claimed, 0, 64);
+      const int nclaimed = __popc(claimed);
+      const int base_c = (chunk * 64 + l) * 32;
+      while (claimed) {
+        const int b = __ffs(claimed) - 1;
+        claimed &= claimed - 1u;
+        async_process_constraint(net, dom, size, base_c + b, F, ctl, revs, dels);
+      }
+      if (nclaimed && lane == 0) atomicSub(&ctl->pending, (unsigned long long)nclaimed);
+      seen = 0;
+      continue;
+    }
+    chunk = chunk + 1 == nchunks ? 0 : chunk + 1;
+    if (++seen >= nchunks) {
+      seen = 0;
+      const unsigned long long p =
+          __hip_atomic_load(&ctl->pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (p == 0ull) break;
+      if (wall_clock64() - t0 > max_ticks) {
+        if (lane == 0) atomicExch(&ctl->err, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(8);
+    }
+  }
+  if (lane == 0 && (revs | dels)) {
+    atomicAdd(&ctl->revisions, revs);
+    atomicAdd(&ctl->deletions, dels);
+  }
+}
+
+// Seed the asynchronous frontier: all active constraints, or those of var.
+__global__ void k_seed_async(unsigned* F, int nfw, int ncons, const int* adj_off, const int4* adj, int var,
+                             AsyncCtl* ctl) {
+  if (var < 0) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nfw) {
+      const int k = ncons - (int)(i * 32);
+      F[i] = k >= 32 ? 0xffffffffu : (k <= 0 ? 0u : ((1u << k) - 1u));
+    }
+    if (i == 0) ctl->pending = (unsigned long long)ncons;
+    return;
+  }
+  if (blockIdx.x != 0) return;
+  for (int i = adj_off[var] + threadIdx.x; i < adj_off[var + 1]; i += blockDim.x) {
+    const int c = adj[i].x;
+    const unsigned bit = 1u << (c & 31);
+    if (!(atomicOr(&F[c >> 5], bit) & bit)) atomicAdd(&ctl->pending, 1ull);
+  }
+}
+
 // Recompute exact sizes of one level (after host-side writes).
 __global__ void k_sizes(const u64* dom, int* size, int n, int W) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -274,9 +421,11 @@ __device__ __forceinline__ WgMem wg_carve(char* base, int n, int W) {
 
 __device__ __forceinline__ bool bit_of(const unsigned* bm, int v) { return (bm[v >> 5] >> (v & 31)) & 1u; }
 
-__device__ __forceinline__ void wg_mark(WgMem& mem, WgCtl* ctl, int v, int slot) {
+// Record that v changed in this round: set its bit in the round's next
+// frontier (the swapped pointer, not mem.nxt) and count it once.
+__device__ __forceinline__ void wg_mark(unsigned* nxt, WgCtl* ctl, int v, int slot) {
   const unsigned bit = 1u << (v & 31);
-  const unsigned old = atomicOr(&mem.nxt[v >> 5], bit);
+  const unsigned old = atomicOr(&nxt[v >> 5], bit);
   if (!(old & bit)) atomicAdd(&ctl->nchanged[slot], 1);
 }
 
@@ -327,15 +476,15 @@ __device__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_
     if (nthreads == 64)
       for (int i = lane; i < nvw; i += 64) nxt[i] = 0u;
     __syncthreads();
-    const int total = ctl->total;
-    const int m = ctl->m;
+    const int total = uniform_int(ctl->total);
+    const int m = uniform_int(ctl->m);
     const bool both = both_dirs_round0 && round == 0;
     while (true) {
       int idx = 0;
       if (lane == 0) idx = atomicAdd(&ctl->arc_ctr, 1);
-      idx = __shfl(idx, 0, 64);
+      idx = uniform_int(__shfl(idx, 0, 64));
       if (idx >= total) break;
-      if (ctl->wipe) break;
+      if (uniform_int(ctl->wipe)) break;
       int lo = 0, hi = m - 1;
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -351,19 +500,19 @@ __device__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_
         if (!bit_of(mem.asg, u)) {
           const int r = revise_binary(net, mem.dom, mem.size, c, u, t, e.z, &wiped);
           ++revs;
-          if (r && lane == 0) wg_mark(mem, ctl, u, slot);
+          if (r && lane == 0) wg_mark(nxt, ctl, u, slot);
           if (wiped && lane == 0) ctl->wipe = 1;
         }
         if (both && !bit_of(mem.asg, t)) {
           const int r = revise_binary(net, mem.dom, mem.size, c, t, u, 1 - e.z, &wiped);
           ++revs;
-          if (r && lane == 0) wg_mark(mem, ctl, t, slot);
+          if (r && lane == 0) wg_mark(nxt, ctl, t, slot);
           if (wiped && lane == 0) ctl->wipe = 1;
         }
       } else {
         revise_table(net, mem.dom, mem.size, c, both ? -1 : t, mem.asg,
                      [&](int v, int) {
-                       if (lane == 0) wg_mark(mem, ctl, v, slot);
+                       if (lane == 0) wg_mark(nxt, ctl, v, slot);
                      },
                      &wiped);
         ++revs;
@@ -371,11 +520,11 @@ __device__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_
       }
     }
     __syncthreads();
-    if (ctl->wipe) {
+    if (uniform_int(ctl->wipe)) {
       result = false;
       break;
     }
-    if (ctl->nchanged[slot] == 0) break;
+    if (uniform_int(ctl->nchanged[slot]) == 0) break;
     unsigned* tmp = cur;
     cur = nxt;
     nxt = tmp;
@@ -421,7 +570,7 @@ __device__ __forceinline__ u64 wg_reduce_min_u64(WgCtl* ctl, u64 v) {
   u64 r = ctl->red[0];
   for (int i = 1; i < nwaves; ++i) r = ctl->red[i] < r ? ctl->red[i] : r;
   __syncthreads();
-  return r;
+  return (u64)uniform_ll((long long)r);
 }
 
 __device__ __forceinline__ u64 wg_reduce_sum_u64(WgCtl* ctl, u64 v) {
@@ -432,7 +581,7 @@ __device__ __forceinline__ u64 wg_reduce_sum_u64(WgCtl* ctl, u64 v) {
   u64 r = 0;
   for (int i = 0; i < nwaves; ++i) r += ctl->red[i];
   __syncthreads();
-  return r;
+  return (u64)uniform_ll((long long)r);
 }
 
 __device__ __forceinline__ u64 wg_dom_hash(WgCtl* ctl, const u64* dom, int n, int W) {
@@ -489,47 +638,46 @@ __global__ void k_block_gac(DevNet net, u64* dom, int* size, int seed_var, char*
 }
 
 // ---------------------------------------------------------------------------
-// Batched singleton tests: every workgroup repeatedly takes a test (x, a)
-// from a global counter, copies the base level into its domain copy, reduces
-// x to {a}, marks x assigned and runs the fixpoint from {x}; fail[i] records
-// a wipe-out.  The copies never touch HBM when they fit in LDS.
+// Batched singleton tests: workgroup `first + blockIdx.x` takes test (x, a),
+// copies the base level into its domain copy, reduces x to {a}, marks x
+// assigned and runs the fixpoint from {x}; fail[i] records a wipe-out.  The
+// copies never touch HBM when they fit in LDS.  One test per workgroup: the
+// kernel has no loop around its barriers, and the hardware dispatcher keeps
+// every CU busy as workgroups finish at different times.
 // ---------------------------------------------------------------------------
 template <bool kLds>
 __global__ void k_singleton_tests(DevNet net, const u64* __restrict__ base_dom,
                                   const int* __restrict__ base_size, const int2* __restrict__ tests,
-                                  long long ntests, unsigned char* fail, unsigned long long* counter,
+                                  long long ntests, long long first, unsigned char* fail,
                                   unsigned long long* revs_out, char* scratch, long long scratch_stride) {
   extern __shared__ __align__(16) char smem[];
   __shared__ WgCtl ctl;
-  __shared__ long long unit_sh;
+  const long long unit = first + blockIdx.x;
+  if (unit >= ntests) return;
   const int n = net.n, W = net.W;
   const int nvw = (n + 31) / 32;
   char* base = kLds ? smem : scratch + (long long)blockIdx.x * scratch_stride;
   WgMem mem = wg_carve(base, n, W);
+  const int2 tst = tests[unit];
+  const int x = uniform_int(tst.x), a = uniform_int(tst.y);
+  wg_copy_words(mem.dom, base_dom, (long long)n * W);
+  for (int v = threadIdx.x; v < n; v += blockDim.x) mem.size[v] = base_size[v];
+  wg_clear_bitmap(mem.asg, nvw);
+  wg_clear_bitmap(mem.cur, nvw);
   if (threadIdx.x == 0) ctl.revs = 0;
-  while (true) {
-    if (threadIdx.x == 0) unit_sh = (long long)atomicAdd(counter, 1ull);
-    __syncthreads();
-    const long long unit = unit_sh;
-    if (unit >= ntests) break;
-    const int2 tst = tests[unit];
-    const int x = tst.x, a = tst.y;
-    wg_copy_words(mem.dom, base_dom, (long long)n * W);
-    for (int v = threadIdx.x; v < n; v += blockDim.x) mem.size[v] = base_size[v];
-    wg_clear_bitmap(mem.asg, nvw);
-    wg_clear_bitmap(mem.cur, nvw);
-    __syncthreads();
-    if (threadIdx.x < W) mem.dom[(long long)x * W + threadIdx.x] = (threadIdx.x == (a >> 6)) ? (1ull << (a & 63)) : 0ull;
-    if (threadIdx.x == 0) {
-      mem.size[x] = 1;
-      mem.asg[x >> 5] |= 1u << (x & 31);
-      mem.cur[x >> 5] |= 1u << (x & 31);
-    }
-    __syncthreads();
-    const bool ok = wg_fixpoint(net, mem, &ctl, false);
-    if (threadIdx.x == 0) fail[unit] = ok ? 0 : 1;
+  __syncthreads();
+  if (threadIdx.x < W) mem.dom[(long long)x * W + threadIdx.x] = (threadIdx.x == (a >> 6)) ? (1ull << (a & 63)) : 0ull;
+  if (threadIdx.x == 0) {
+    mem.size[x] = 1;
+    mem.asg[x >> 5] |= 1u << (x & 31);
+    mem.cur[x >> 5] |= 1u << (x & 31);
   }
-  if (threadIdx.x == 0 && ctl.revs) atomicAdd(revs_out, ctl.revs);
+  __syncthreads();
+  const bool ok = wg_fixpoint(net, mem, &ctl, false);
+  if (threadIdx.x == 0) {
+    fail[unit] = ok ? 0 : 1;
+    if (ctl.revs) atomicAdd(revs_out, ctl.revs);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -577,7 +725,7 @@ enum { MAC_INIT = 0, MAC_RUNNING = 1, MAC_SAT = 2, MAC_UNSAT = 3, MAC_LIMIT = 4 
 __device__ void wg_trace(const DevNet& net, WgMem mem, WgCtl* ctl, const MacArgs& A, int positive,
                          int var, int val, bool ok) {
   if (!A.trace) return;
-  const int tl = ctl->trace_len;
+  const int tl = uniform_int(ctl->trace_len);
   if (tl < A.trace_cap) {
     const u64 h = ok ? wg_dom_hash(ctl, mem.dom, net.n, net.W) : 0ull;
     if (threadIdx.x == 0) {
@@ -599,17 +747,23 @@ __device__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* ctl, const MacAr
   const int n = net.n, W = net.W;
   const int nvw = (n + 31) / 32;
   const long long nw = (long long)n * W;
+  // Search control lives in registers, identical in every thread (all values
+  // below are wave-uniform); LDS only carries what thread 0 publishes.
+  int status = A.state->status;
+  int depth = A.state->depth;
+  long long npos = A.state->npos;
+  long long nneg = A.state->nneg;
+  long long nodes_here = 0;
   if (threadIdx.x == 0) {
-    ctl->status = A.state->status;
-    ctl->depth = A.state->depth;
-    ctl->npos = A.state->npos;
-    ctl->nneg = A.state->nneg;
     ctl->trace_len = A.state->trace_len;
-    ctl->nodes_here = 0;
     ctl->revs = 0;
   }
+  status = uniform_int(status);
+  depth = uniform_int(depth);
+  npos = uniform_ll(npos);
+  nneg = uniform_ll(nneg);
   __syncthreads();
-  if (ctl->status == MAC_INIT) {
+  if (status == MAC_INIT) {
     wg_copy_words(mem.dom, A.trail, nw);
     wg_clear_bitmap(mem.asg, nvw);
     __syncthreads();
@@ -619,13 +773,10 @@ __device__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* ctl, const MacAr
     const bool ok = wg_fixpoint(net, mem, ctl, false);
     if (ok) wg_copy_words(A.trail, mem.dom, nw);
     if (A.root_out) wg_copy_words(A.root_out, mem.dom, nw);
-    if (threadIdx.x == 0) {
-      ctl->status = ok ? MAC_RUNNING : MAC_UNSAT;
-      ctl->depth = 0;
-    }
+    status = ok ? MAC_RUNNING : MAC_UNSAT;
+    depth = 0;
     __syncthreads();
-  } else if (ctl->status == MAC_RUNNING) {
-    const int depth = ctl->depth;
+  } else if (status == MAC_RUNNING) {
     wg_copy_words(mem.dom, A.trail + depth * nw, nw);
     wg_clear_bitmap(mem.asg, nvw);
     __syncthreads();
@@ -634,23 +785,14 @@ __device__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* ctl, const MacAr
       for (int i = 0; i < depth; ++i) mem.asg[A.stack[i].x >> 5] |= 1u << (A.stack[i].x & 31);
     __syncthreads();
   }
-  while (ctl->status == MAC_RUNNING) {
-    if (A.node_limit > 0 && ctl->npos + ctl->nneg >= A.node_limit) {
+  while (status == MAC_RUNNING) {
+    const bool over_nodes = A.node_limit > 0 && npos + nneg >= A.node_limit;
+    const bool over_time = A.deadline && uniform_ll((long long)wall_clock64()) > (long long)A.deadline;
+    if (over_nodes || over_time || (A.step_budget > 0 && nodes_here >= A.step_budget)) {
+      // suspend (step budget) or stop (limits) at a consistent state of depth `depth`
+      wg_copy_words(A.trail + depth * nw, mem.dom, nw);
+      if (over_nodes || over_time) status = MAC_LIMIT;
       __syncthreads();
-      if (threadIdx.x == 0) ctl->status = MAC_LIMIT;
-      __syncthreads();
-      wg_copy_words(A.trail + ctl->depth * nw, mem.dom, nw);
-      break;
-    }
-    if (A.deadline && wall_clock64() > A.deadline) {
-      __syncthreads();
-      if (threadIdx.x == 0) ctl->status = MAC_LIMIT;
-      __syncthreads();
-      wg_copy_words(A.trail + ctl->depth * nw, mem.dom, nw);
-      break;
-    }
-    if (A.step_budget > 0 && ctl->nodes_here >= A.step_budget) {
-      wg_copy_words(A.trail + ctl->depth * nw, mem.dom, nw);
       break;
     }
     // select: min |D| over undecided variables, lowest id on ties
@@ -662,22 +804,15 @@ __device__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* ctl, const MacAr
       }
     key = wg_reduce_min_u64(ctl, key);
     const int x = (int)(key & 0xffffffffu);
-    int depth = ctl->depth;
-    if (threadIdx.x == 0) {
-      int a = -1;
-      for (int w = 0; w < W; ++w) {
-        const u64 dw = mem.dom[(long long)x * W + w];
-        if (dw) {
-          a = w * 64 + (int)__builtin_ctzll(dw);
-          break;
-        }
-      }
-      ctl->sel_var = x;
-      ctl->sel_val = a;
+    // min value of x (every thread scans the same LDS words)
+    int a = -1;
+    for (int w = 0; w < W && a < 0; ++w) {
+      const u64 dw = mem.dom[(long long)x * W + w];
+      if (dw) a = w * 64 + (int)__builtin_ctzll(dw);
     }
+    a = uniform_int(a);
     wg_copy_words(A.trail + depth * nw, mem.dom, nw);
     __syncthreads();
-    const int a = ctl->sel_val;
     if (threadIdx.x < W) mem.dom[(long long)x * W + threadIdx.x] = (threadIdx.x == (a >> 6)) ? (1ull << (a & 63)) : 0ull;
     wg_clear_bitmap(mem.cur, nvw);
     __syncthreads();
@@ -686,71 +821,65 @@ __device__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* ctl, const MacAr
       mem.size[x] = 1;
       mem.asg[x >> 5] |= 1u << (x & 31);
       mem.cur[x >> 5] |= 1u << (x & 31);
-      ctl->depth = depth + 1;
-      ctl->npos += 1;
-      ctl->nodes_here += 1;
     }
+    ++depth;
+    ++npos;
+    ++nodes_here;
     __syncthreads();
     bool ok = wg_fixpoint(net, mem, ctl, false);
     wg_trace(net, mem, ctl, A, 1, x, a, ok);
-    depth = ctl->depth;
     if (ok && depth == n) {
       if (A.solution)
         for (int v = threadIdx.x; v < n; v += blockDim.x) {
-          int s = -1;
-          for (int w = 0; w < W; ++w) {
+          int sv = -1;
+          for (int w = 0; w < W && sv < 0; ++w) {
             const u64 dw = mem.dom[(long long)v * W + w];
-            if (dw) {
-              s = w * 64 + (int)__builtin_ctzll(dw);
-              break;
-            }
+            if (dw) sv = w * 64 + (int)__builtin_ctzll(dw);
           }
-          A.solution[v] = s;
+          A.solution[v] = sv;
         }
-      __syncthreads();
-      if (threadIdx.x == 0) ctl->status = MAC_SAT;
+      status = MAC_SAT;
       __syncthreads();
       break;
     }
     while (!ok && depth > 0) {
       --depth;
       const int2 d = A.stack[depth];
+      const int dx = uniform_int(d.x), dv = uniform_int(d.y);
       wg_copy_words(mem.dom, A.trail + depth * nw, nw);
       __syncthreads();
       wg_recount(mem, n, W);
       __syncthreads();
       if (threadIdx.x == 0) {
-        mem.asg[d.x >> 5] &= ~(1u << (d.x & 31));
-        mem.dom[(long long)d.x * W + (d.y >> 6)] &= ~(1ull << (d.y & 63));
-        mem.size[d.x] -= 1;
-        ctl->depth = depth;
-        ctl->nneg += 1;
-        ctl->nodes_here += 1;
+        mem.asg[dx >> 5] &= ~(1u << (dx & 31));
+        mem.dom[(long long)dx * W + (dv >> 6)] &= ~(1ull << (dv & 63));
+        mem.size[dx] -= 1;
       }
+      ++nneg;
+      ++nodes_here;
       wg_clear_bitmap(mem.cur, nvw);
       __syncthreads();
-      if (mem.size[d.x] == 0) {
+      if (uniform_int(mem.size[dx]) == 0) {
         ok = false;
       } else {
-        if (threadIdx.x == 0) mem.cur[d.x >> 5] |= 1u << (d.x & 31);
+        if (threadIdx.x == 0) mem.cur[dx >> 5] |= 1u << (dx & 31);
         __syncthreads();
         ok = wg_fixpoint(net, mem, ctl, false);
       }
-      wg_trace(net, mem, ctl, A, 0, d.x, d.y, ok);
+      wg_trace(net, mem, ctl, A, 0, dx, dv, ok);
     }
     if (!ok) {
-      __syncthreads();
-      if (threadIdx.x == 0) ctl->status = MAC_UNSAT;
+      status = MAC_UNSAT;
       __syncthreads();
       break;
     }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    A.state->status = ctl->status;
-    A.state->depth = ctl->depth;
-    A.state->npos = ctl->npos;
-    A.state->nneg = ctl->nneg;
+    A.state->status = status;
+    A.state->depth = depth;
+    A.state->npos = npos;
+    A.state->nneg = nneg;
     A.state->revisions += (long long)ctl->revs;
     A.state->trace_len = ctl->trace_len;
     A.state->steps += 1;
@@ -787,7 +916,7 @@ __global__ void k_mac_units(const UnitDesc* __restrict__ units, int count, unsig
   while (true) {
     if (threadIdx.x == 0) unit_sh = only >= 0 ? only : (int)atomicAdd(counter, 1u);
     __syncthreads();
-    const int u = unit_sh;
+    const int u = uniform_int(unit_sh);
     __syncthreads();
     if (u >= count) break;
     const UnitDesc d = units[u];

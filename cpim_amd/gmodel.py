@@ -134,12 +134,19 @@ class GModel:
         return st
 
     def EnforceGAC_Persistent(self, verbose: bool = True, assigned_var: int = -1) -> GacStats:
-        st = self._gac(assigned_var, N.CPIM_GAC_PERSISTENT)
+        """All rounds in one launch.  Served by the asynchronous one-launch
+        kernel (no grid barrier); the cooperative grid-barrier kernel is
+        switched off in the engine."""
+        st = self._gac(assigned_var, N.CPIM_GAC_ASYNC)
         if verbose:
             print(f"[GAC-Persistent] iterations={st.iterations} deletions={st.deletions} "
                   f"inconsistent={'true' if st.inconsistent else 'false'} "
                   f"converged={'true' if st.converged else 'false'}")
         return st
+
+    def EnforceGAC_Async(self, assigned_var: int = -1) -> GacStats:
+        """Whole GPU, one launch, asynchronous work queue."""
+        return self._gac(assigned_var, N.CPIM_GAC_ASYNC)
 
     def EnforceGAC_Block(self, assigned_var: int = -1) -> GacStats:
         """One workgroup, domains in LDS (one launch)."""

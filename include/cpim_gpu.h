@@ -141,9 +141,12 @@ typedef struct cpim_gac_stats {
 } cpim_gac_stats;
 
 #define CPIM_GAC_PLAIN 0      /* host-driven rounds, one launch + sync per round */
-#define CPIM_GAC_PERSISTENT 1 /* all rounds in one cooperative launch */
+#define CPIM_GAC_PERSISTENT 1 /* all rounds in one cooperative launch (grid barrier);
+                                 currently switched off: returns CPIM_ERR_UNSUPPORTED */
 #define CPIM_GAC_BLOCK 2      /* one workgroup, domains in LDS, one launch */
-#define CPIM_GAC_AUTO 3       /* autotuned choice among the above */
+#define CPIM_GAC_AUTO 3       /* autotuned choice among the modes that run */
+#define CPIM_GAC_ASYNC 4      /* one launch, whole GPU, asynchronous work queue,
+                                 no grid barrier */
 
 /* GModel::EnforceGAC / EnforceGAC_Persistent (include/GModel.cuh:191-197,
  * src/GModel.cu:1110-1391) on the current level.  assigned_var < 0 seeds all
