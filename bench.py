@@ -39,11 +39,28 @@ def parse():
                     help="singleton tests timed on the CPU oracle (about 10-15 s)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (read MIB + write MIB) and exit: the "
+                         "reference kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
+
+
+def calibrate_copy(mib: int) -> None:
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": 4 * n, "read_bytes": 4 * n, "write_bytes": 4 * n}))
 
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is design-only, see DESIGN.md", file=sys.stderr)
