@@ -9,7 +9,10 @@ domain copy, all in one persistent launch.  The plain path (one host-driven
 fixpoint per test, one launch + sync per round) and the CPU oracle are timed
 on bounded samples of the same tests in the same run.
 
-Prints one JSON line (see DESIGN.md for the roofline definitions).
+Prints one JSON line.  roofline.achieved counts the schedule-independent
+algorithmic bytes of each test's seed revisions (8 W (4 deg(x) + n) per test);
+roofline.traffic is the measured HBM bytes per launch from the committed
+rocprofv3 counter summary under profiles/.
 """
 from __future__ import annotations
 
@@ -45,15 +48,35 @@ def parse():
     return ap.parse_args()
 
 
+PROFILE = os.path.join(REPO, "profiles", "batched_sac_singleton_tests.txt")
+
+
+def profiled_traffic() -> float | None:
+    """HBM bytes per k_singleton_tests launch on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel calibration)."""
+    try:
+        with open(PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_bandwidth(mib: int, reps: int) -> float:
+    """Best streaming-copy bandwidth (GB/s, read + write) of k_stream_copy."""
+    import ctypes as C
+    from cpim_amd import _native as N
+    gbps = C.c_double(0)
+    N.check(N.gpu_lib().cpim_measure_copy_bandwidth(0, mib << 20, reps, C.byref(gbps)))
+    return gbps.value
+
+
 def calibrate_copy(mib: int) -> None:
-    import torch
-    n = mib * (1 << 20) // 4
-    src = torch.ones(n, dtype=torch.int32, device="cuda")
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"calibration_copy_bytes": 4 * n, "read_bytes": 4 * n, "write_bytes": 4 * n}))
+    bw = copy_bandwidth(mib, 1)
+    print(json.dumps({"calibration_copy_bytes": mib << 20, "read_bytes": mib << 20,
+                      "write_bytes": mib << 20, "gbps": bw}))
 
 
 def main():
@@ -63,7 +86,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is design-only, see DESIGN.md", file=sys.stderr)
+              "singleton tests would be split across GPUs with one OR all-reduce of removal masks per SAC pass; that path is not built", file=sys.stderr)
         sys.exit(2)
     from cpim_amd import GModelAdapter, workload
     from cpim_amd.model import domains_to_sets
@@ -161,8 +184,13 @@ def main():
         "plain_fixpoints_per_s": plain,
         "speedup_vs_plain": (value / plain) if plain else None,
         "kernel_ms_per_step": avg_kernel_ms,
+        "stream_copy_gbps": copy_bandwidth(1024, 5),
         "revisions_per_s": st.revisions / (avg_kernel_ms * 1e-3),
     }
+    traffic = profiled_traffic()
+    out["roofline"]["traffic"] = traffic
+    if traffic:
+        out["measured_dram_gbps"] = traffic / (avg_kernel_ms * 1e-3) / 1e9
     m.close()
     print(json.dumps(out), flush=True)
 

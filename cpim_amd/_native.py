@@ -181,6 +181,7 @@ GPU_SIGNATURES = {
     "cpim_last_error": (C.c_char_p, []),
     "cpim_version": (C.c_char_p, []),
     "cpim_get_device_info": (C.c_int, [C.c_int32, C.POINTER(CDeviceInfo)]),
+    "cpim_measure_copy_bandwidth": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "cpim_gmodel_create": (C.c_int, [C.POINTER(CModelDesc), C.POINTER(CGModelOptions), C.POINTER(_VP)]),
     "cpim_gmodel_destroy": (C.c_int, [_VP]),
     "cpim_gmodel_get_info": (C.c_int, [_VP, C.POINTER(CGModelInfo)]),

@@ -445,6 +445,39 @@ int cpim_get_device_info(int32_t device_id, cpim_device_info* out) {
   });
 }
 
+int cpim_measure_copy_bandwidth(int32_t device_id, int64_t bytes, int32_t reps, double* gbps) {
+  return guarded([&] {
+    REQUIRE(gbps && bytes >= 16 && reps >= 1, "bad argument");
+    DeviceGuard g(device_id);
+    const DevProps p = props_of(device_id);
+    const long long n16 = bytes / 16;
+    DevAlloc a(16 * n16), b(16 * n16);
+    hipStream_t s;
+    HIPC(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIPC(hipMemsetAsync(a.p, 1, 16 * n16, s));
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    const int threads = 256;
+    const long long blocks = std::min<long long>((n16 + threads - 1) / threads, 8ll * p.cus * 4);
+    double best = 0;
+    for (int r = 0; r < reps; ++r) {
+      HIPC(hipEventRecord(e0, s));
+      hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(threads), 0, s, a.as<uint4>(), b.as<uint4>(), n16);
+      HIPC(hipGetLastError());
+      HIPC(hipEventRecord(e1, s));
+      HIPC(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPC(hipEventElapsedTime(&ms, e0, e1));
+      if (ms > 0) best = std::max(best, 2.0 * 16 * n16 / (ms * 1e-3) / 1e9);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    *gbps = best;
+  });
+}
+
 int cpim_gmodel_create(const cpim_model_desc* desc, const cpim_gmodel_options* opts,
                        cpim_gmodel** out) {
   return guarded([&] {

@@ -342,6 +342,14 @@ __global__ void k_seed_async(unsigned* F, int nfw, int ncons, const int* adj_off
   }
 }
 
+// Streaming copy (16-byte lanes, grid-stride): the known-traffic kernel used
+// to calibrate hardware byte counters and to measure achievable bandwidth.
+__global__ void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, long long n16) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // Recompute exact sizes of one level (after host-side writes).
 __global__ void k_sizes(const u64* dom, int* size, int n, int W) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;

@@ -101,6 +101,13 @@ typedef struct cpim_device_info {
 
 int cpim_get_device_info(int32_t device_id, cpim_device_info* out);
 
+/* Measurement utility (no counterpart in the reference): copy `bytes` bytes
+ * between two device buffers with a streaming kernel, `reps` times, and return
+ * the best achieved bandwidth (read + write bytes per second, GB/s).  It is
+ * the reference kernel of known traffic for scaling hardware counters and the
+ * measured stream-copy peak reported next to the 8 TB/s vendor figure. */
+int cpim_measure_copy_bandwidth(int32_t device_id, int64_t bytes, int32_t reps, double* gbps);
+
 /* ------------------------------------------------------------------------ */
 /* GModel: device-resident model, domain level stack and propagation         */
 /* ------------------------------------------------------------------------ */
