@@ -42,6 +42,13 @@ def parse():
                     help="singleton tests timed on the CPU oracle (about 10-15 s)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-extras", action="store_true",
+                    help="skip the single-fixpoint, uneven-batch and serving measurements")
+    ap.add_argument("--batch-instances", type=int, default=512,
+                    help="instances of the 4096-instance uneven batch timed in the extras")
+    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--serve-rate", type=float, default=2000.0,
+                    help="Poisson arrival rate of serving requests (requests/s)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (read MIB + write MIB) and exit: the "
                          "reference kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE counters")
@@ -77,6 +84,107 @@ def calibrate_copy(mib: int) -> None:
     bw = copy_bandwidth(mib, 1)
     print(json.dumps({"calibration_copy_bytes": mib << 20, "read_bytes": mib << 20,
                       "write_bytes": mib << 20, "gbps": bw}))
+
+
+def single_fixpoint(log) -> dict:
+    """One root GAC fixpoint of n=2000 d=64 e=40000 (10% of values pre-removed), per mode."""
+    from cpim_amd import GModelAdapter, workload
+    desc = workload.single_large()
+    out = {"workload": "root GAC fixpoint, random binary CSP n=2000 d=64 e=40000 tightness 0.25, "
+                       "10% of values pre-removed"}
+    with GModelAdapter.Build(desc) as m:
+        init = m.ReadDomains(0)
+        for name, fn in (("plain", m.EnforceGAC), ("async", m.EnforceGAC_Async),
+                         ("block", m.EnforceGAC_Block)):
+            times = []
+            for rep in range(4):
+                m.WriteDomains(0, init)
+                t = time.perf_counter()
+                st = fn() if name != "plain" else fn(False)
+                dt = time.perf_counter() - t
+                if rep:
+                    times.append(dt)
+            out[f"{name}_ms"] = 1000.0 * float(np.median(times))
+            out[f"{name}_revisions"] = int(st.revisions)
+        out["deletions"] = int(st.deletions)
+        out["inconsistent"] = bool(st.inconsistent)
+    log(f"single fixpoint: plain {out['plain_ms']:.2f} ms, async {out['async_ms']:.2f} ms, "
+        f"block {out['block_ms']:.2f} ms")
+    return out
+
+
+def uneven_batch(count: int, log) -> dict:
+    """Root fixpoint + first 64 MAC nodes of `count` instances of the uneven batch."""
+    from cpim_amd import Batch, workload
+    t = time.perf_counter()
+    models = [workload.uneven_instance(i) for i in range(count)]
+    log(f"uneven batch: {count} instances generated in {time.perf_counter() - t:.1f} s")
+    b = Batch(models)
+    try:
+        b.run(64)  # warm-up
+        t = time.perf_counter()
+        res, kms = b.run(64, persistent=True)
+        one = time.perf_counter() - t
+        t = time.perf_counter()
+        b.run(64, persistent=False)
+        per = time.perf_counter() - t
+    finally:
+        b.close()
+    nodes = sum(r.num_positive + r.num_negative for r in res)
+    out = {"workload": f"{count} of the 4096 uneven instances (n~U[20,200], d in {{8..128}}, "
+                       "density~U[0.1,0.5], phase-transition tightness): root fixpoint + first 64 "
+                       "MAC nodes each",
+           "instances": count, "nodes": int(nodes),
+           "one_launch_ms": 1000.0 * one, "per_instance_ms": 1000.0 * per,
+           "fixpoints_per_s": (count + nodes) / one,
+           "speedup_vs_per_instance": per / one}
+    log(f"uneven batch: one launch {out['one_launch_ms']:.1f} ms, per instance "
+        f"{out['per_instance_ms']:.1f} ms, {out['fixpoints_per_s']:,.0f} fixpoints/s")
+    return out
+
+
+def serving(count: int, rate: float, log) -> dict:
+    """Poisson arrivals of MAC solve requests (binary part + arity-3 tables)."""
+    from cpim_amd import Server, workload
+    node_limit = 5000
+    reqs = [workload.serving_request(i) for i in range(count)]
+    rng = np.random.default_rng(0x5EED2026)
+    arrive = np.cumsum(rng.exponential(1.0 / rate, count))
+    srv = Server(max_slots=256, nodes_per_step=64)
+    lat, limited, solved = [], 0, 0
+    try:
+        pending = {}
+        nxt = 0
+        t0 = time.perf_counter()
+        while nxt < count or pending:
+            now = time.perf_counter() - t0
+            while nxt < count and arrive[nxt] <= now:
+                pending[srv.submit(reqs[nxt], node_limit=node_limit)] = arrive[nxt]
+                nxt += 1
+            if not pending:
+                time.sleep(max(0.0, arrive[nxt] - now))
+                continue
+            srv.step()
+            done_at = time.perf_counter() - t0
+            for rid in list(pending):
+                r = srv.result(rid)
+                if r is not None:
+                    lat.append(done_at - pending.pop(rid))
+                    limited += r["status"] == 0
+                    solved += r["status"] == 1
+        wall = time.perf_counter() - t0
+    finally:
+        srv.close()
+    lat_ms = 1000.0 * np.array(lat)
+    out = {"workload": f"{count} requests, Poisson arrivals at {rate:g}/s; each a MAC solve "
+                       f"(node limit {node_limit}) of n~U[20,40] d=16 binary CSP + 4 arity-3 "
+                       "tables of 500 tuples",
+           "solves_per_s": count / wall, "p50_ms": float(np.percentile(lat_ms, 50)),
+           "p99_ms": float(np.percentile(lat_ms, 99)), "solved": int(solved),
+           "node_limited": int(limited)}
+    log(f"serving: {out['solves_per_s']:.0f} solves/s, p50 {out['p50_ms']:.1f} ms, "
+        f"p99 {out['p99_ms']:.1f} ms")
+    return out
 
 
 def main():
@@ -187,6 +295,15 @@ def main():
         "stream_copy_gbps": copy_bandwidth(1024, 5),
         "revisions_per_s": st.revisions / (avg_kernel_ms * 1e-3),
     }
+    if not args.skip_extras:
+        for key, fn in (("single_fixpoint", lambda: single_fixpoint(log)),
+                        ("uneven_batch", lambda: uneven_batch(args.batch_instances, log)),
+                        ("serving", lambda: serving(args.serve_requests, args.serve_rate, log))):
+            try:
+                out[key] = fn()
+            except Exception as exc:  # reported in the line, never hidden
+                log(f"{key} failed: {exc!r}")
+                out[key] = {"error": repr(exc)}
     traffic = profiled_traffic()
     out["roofline"]["traffic"] = traffic
     if traffic:

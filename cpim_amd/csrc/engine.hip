@@ -680,8 +680,7 @@ void gac_block(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
 
 int auto_gac_mode(const cpim_gmodel* m) {
   // Small networks: one workgroup with the domains in LDS (one launch, no
-  // grid barrier).  Large ones: the persistent grid when cooperative launch
-  // is available.
+  // grid barrier).  Large ones: the asynchronous one-launch grid fixpoint.
   if (fits_lds(m) && m->dm.num_active <= 4096) return CPIM_GAC_BLOCK;
   return CPIM_GAC_ASYNC;
 }

@@ -1,0 +1,112 @@
+"""Batched instances and the serving layer vs the oracle (needs an MI355X).
+
+Uneven batch: every instance's root fixpoint domains and its first 64 MAC
+nodes (decision, value, outcome and domain hash of each node) must equal the
+oracle's MAC run on that instance alone, whether all instances share one
+launch or run one launch each.  Serving: every request's result (status,
+node counts, solution) is the same as solving it alone, whatever else is
+resident, and the same as the oracle's.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cpim_amd import Batch, GModelAdapter, GModelOptions, Server, workload
+
+pytestmark = pytest.mark.gpu
+
+NODES = 64
+
+
+def oracle_status(r, node_limit):
+    if r.num_sol:
+        return 1
+    return 0 if r.time_out else 2
+
+
+@pytest.fixture(scope="module")
+def uneven():
+    # 40 instances of the 4096-instance uneven batch (n in [20, 200], d in {8..128})
+    idx = list(range(0, 4096, 103))
+    models = [workload.uneven_instance(i) for i in idx]
+    want = []
+    for m in models:
+        o = oracle.Oracle(m)
+        ok, root, _ = o.root_ac()
+        r = oracle.Oracle(m).mac(node_limit=NODES, trace_cap=2 * NODES)
+        want.append((ok, root, r))
+    return models, want
+
+
+@pytest.mark.parametrize("persistent", [True, False], ids=["one-launch", "per-instance"])
+def test_uneven_batch_first_64_nodes(gpu, uneven, persistent):
+    models, want = uneven
+    b = Batch(models)
+    try:
+        res, ms = b.run(NODES, persistent=persistent, trace_capacity=2 * NODES)
+        assert ms > 0
+        for i, (m, (ok, root, r), got) in enumerate(zip(models, want, res)):
+            assert got.root_consistent == ok, i
+            if ok:
+                assert (b.root_domains(i) == root).all(), i
+            assert (got.num_positive, got.num_negative) == (r.num_positive, r.num_negative), i
+            assert got.status == oracle_status(r, NODES), i
+            assert got.trace == r.trace, i
+    finally:
+        b.close()
+
+
+def test_uneven_batch_rerun_is_identical(gpu, uneven):
+    models, _ = uneven
+    b = Batch(models[:12])
+    try:
+        first, _ = b.run(NODES, trace_capacity=2 * NODES)
+        again, _ = b.run(NODES, trace_capacity=2 * NODES)
+        assert [(x.status, x.num_positive, x.num_negative, x.trace) for x in first] == \
+               [(x.status, x.num_positive, x.num_negative, x.trace) for x in again]
+    finally:
+        b.close()
+
+
+def solve_alone(desc, node_limit):
+    with GModelAdapter.Build(desc, GModelOptions(skip_non_binary=False)) as m:
+        st, sol, _ = m.Solve(node_limit=node_limit)
+    status = 1 if st.num_solutions else (0 if st.time_out else 2)
+    return status, st.num_positive, st.num_negative, (list(sol) if st.num_solutions else None)
+
+
+@pytest.mark.parametrize("nodes_per_step,max_slots", [(64, 256), (7, 5)])
+def test_serving_mix_equals_alone(gpu, nodes_per_step, max_slots):
+    limit = 3000
+    reqs = [workload.serving_request(i) for i in range(16)]
+    srv = Server(max_slots=max_slots, nodes_per_step=nodes_per_step)
+    try:
+        ids = [srv.submit(r, node_limit=limit) for r in reqs]
+        steps = 0
+        while srv.step() > 0:
+            steps += 1
+            assert steps < 100000
+        for rid, desc in zip(ids, reqs):
+            got = srv.result(rid)
+            assert got is not None
+            r = oracle.Oracle(desc, oracle.AC3).mac(node_limit=limit)
+            want = (oracle_status(r, limit), r.num_positive, r.num_negative,
+                    list(r.solution) if r.num_sol else None)
+            have = (got["status"], got["num_positive"], got["num_negative"],
+                    list(got["solution"]) if got["solution"] is not None else None)
+            assert have == want, rid
+            assert solve_alone(desc, limit) == want, rid
+    finally:
+        srv.close()
+
+
+def test_serving_result_busy_until_done(gpu):
+    srv = Server(max_slots=2, nodes_per_step=1)
+    try:
+        rid = srv.submit(workload.serving_request(3), node_limit=500)
+        assert srv.result(rid) is None
+        while srv.step() > 0:
+            pass
+        assert srv.result(rid) is not None
+    finally:
+        srv.close()
