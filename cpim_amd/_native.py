@@ -30,6 +30,8 @@ CPIM_GAC_ASYNC = 4
 
 CPIM_SAC_BATCHED = 0
 CPIM_SAC_PLAIN = 1
+CPIM_SAC_WORKGROUP = 2
+CPIM_SAC_SLICED = 3
 
 CPIM_BRANCH_BINARY = 0
 

@@ -964,6 +964,8 @@ namespace {
 struct SacScratch {
   DevAlloc tests, fail, revs;
   long long cap = 0;
+  DevAlloc stests, perm, gfirst;  // bit-sliced path: sorted tests, their order, group starts
+  long long scap = 0, gcap = 0;
 };
 
 // Launch k_singleton_tests over `count` tests: one workgroup per test.  With
@@ -1020,6 +1022,116 @@ void run_singleton_batched(cpim_gmodel* m, const int2* tests_host, long long cou
   st->kernel_ms += ms;
   st->revisions += (int64_t)revs;
   st->singleton_tests += count;
+}
+
+// Bit-sliced path: tests sorted by variable and cut into groups of at most
+// 64 (a variable with more values is split into balanced groups; small
+// variables share a group), one workgroup per group.  The slices of the
+// resident groups take n * dpad words each, within kSliceBudgetBytes.
+constexpr int kSliceThreads = 256;
+constexpr long long kSliceBudgetBytes = 4ll << 30;
+
+bool sliced_eligible(const cpim_gmodel* m) {
+  return (int)m->im.bin_cons.size() == m->im.num_active &&
+         slice_lds_bytes(m->dm.n, m->dm.dpad, kSliceThreads / 64) <= kLdsBudget;
+}
+
+void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long count, unsigned char* fail_host,
+                          cpim_sac_stats* st, SacScratch& S) {
+  if (count == 0) return;
+  REQUIRE(count <= INT_MAX, "too many singleton tests for one call");
+  std::vector<int> order((size_t)count);
+  for (long long i = 0; i < count; ++i) order[(size_t)i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return tests_host[a].x < tests_host[b].x; });
+  std::vector<int2> sorted((size_t)count);
+  for (long long i = 0; i < count; ++i) sorted[(size_t)i] = tests_host[order[(size_t)i]];
+  std::vector<int> gfirst;
+  int open = -1;  // start of the group being filled, -1 if none
+  for (long long i = 0; i < count;) {
+    long long j = i;
+    while (j < count && sorted[(size_t)j].x == sorted[(size_t)i].x) ++j;
+    const long long len = j - i;
+    if (len > 64) {
+      if (open >= 0) open = -1;
+      const long long k = (len + 63) / 64;
+      for (long long p = 0; p < k; ++p) gfirst.push_back((int)(i + len * p / k));
+    } else {
+      if (open < 0 || i + len - open > 64) {
+        gfirst.push_back((int)i);
+        open = (int)i;
+      }
+    }
+    i = j;
+  }
+  const long long groups = (long long)gfirst.size();
+  gfirst.push_back((int)count);
+  if (count > S.scap) {
+    S.stests = DevAlloc(sizeof(int2) * count);
+    S.perm = DevAlloc(sizeof(int) * count);
+    S.scap = count;
+  }
+  if (count > S.cap) {
+    S.fail = DevAlloc(count);
+    S.tests = DevAlloc(sizeof(int2) * count);
+    S.cap = count;
+  }
+  if (groups + 1 > S.gcap) {
+    S.gfirst = DevAlloc(sizeof(int) * (groups + 1));
+    S.gcap = groups + 1;
+  }
+  if (!S.revs.p) S.revs = DevAlloc(sizeof(unsigned long long));
+  HIPC(hipMemcpyAsync(S.stests.p, sorted.data(), sizeof(int2) * count, hipMemcpyHostToDevice, m->stream));
+  HIPC(hipMemcpyAsync(S.perm.p, order.data(), sizeof(int) * count, hipMemcpyHostToDevice, m->stream));
+  HIPC(hipMemcpyAsync(S.gfirst.p, gfirst.data(), sizeof(int) * (groups + 1), hipMemcpyHostToDevice, m->stream));
+  HIPC(hipMemsetAsync(S.revs.p, 0, sizeof(unsigned long long), m->stream));
+  const long long slice_bytes = 8ll * m->dm.n * m->dm.dpad;
+  const long long per_launch = std::max<long long>(1, std::min<long long>(groups, kSliceBudgetBytes / slice_bytes));
+  u64* slices = reinterpret_cast<u64*>(m->get_scratch(slice_bytes * per_launch));
+  const int L = m->current_level;
+  DevNet net = m->dm.net();
+  const size_t lds = (size_t)slice_lds_bytes(m->dm.n, m->dm.dpad, kSliceThreads / 64);
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  HIPC(hipEventRecord(e0, m->stream));
+  for (long long g0 = 0; g0 < groups; g0 += per_launch) {
+    const long long blocks = std::min(per_launch, groups - g0);
+    hipLaunchKernelGGL(k_sliced_tests, dim3((unsigned)blocks), dim3(kSliceThreads), lds, m->stream, net,
+                       m->level_dom(L), S.stests.as<int2>(), S.gfirst.as<int>(), g0, S.perm.as<int>(),
+                       S.fail.as<unsigned char>(), S.revs.as<unsigned long long>(), slices);
+    HIPC(hipGetLastError());
+  }
+  HIPC(hipEventRecord(e1, m->stream));
+  HIPC(hipMemcpyAsync(fail_host, S.fail.p, count, hipMemcpyDeviceToHost, m->stream));
+  unsigned long long revs = 0;
+  HIPC(hipMemcpyAsync(&revs, S.revs.p, sizeof(revs), hipMemcpyDeviceToHost, m->stream));
+  m->sync();
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  st->kernel_ms += ms;
+  st->revisions += (int64_t)revs;
+  st->singleton_tests += count;
+}
+
+void run_singleton_plain(cpim_gmodel* m, const int2* tests, long long count, unsigned char* fail_host,
+                         cpim_sac_stats* st);
+
+// CPIM_SAC_BATCHED picks the bit-sliced kernel when the network allows it.
+void run_singleton_device(cpim_gmodel* m, int mode, const int2* tests, long long count, unsigned char* fail_out,
+                          cpim_sac_stats* st, SacScratch& S) {
+  if (mode == CPIM_SAC_BATCHED) mode = sliced_eligible(m) ? CPIM_SAC_SLICED : CPIM_SAC_WORKGROUP;
+  if (mode == CPIM_SAC_SLICED) {
+    if (!sliced_eligible(m))
+      fail(CPIM_ERR_UNSUPPORTED, "bit-sliced singleton tests need a binary network whose work lists fit in LDS");
+    run_singleton_sliced(m, tests, count, fail_out, st, S);
+  } else if (mode == CPIM_SAC_WORKGROUP) {
+    run_singleton_batched(m, tests, count, fail_out, st, S);
+  } else {
+    run_singleton_plain(m, tests, count, fail_out, st);
+  }
 }
 
 // One host-driven fixpoint per test on a scratch level: copy the level,
@@ -1091,14 +1203,9 @@ int cpim_gmodel_singleton_tests(cpim_gmodel* m, const int32_t* tests, int64_t co
       REQUIRE(t[i].x >= 0 && t[i].x < m->dm.n, "test names an unknown variable");
       REQUIRE(t[i].y >= 0 && t[i].y < m->im.cap[t[i].x], "test value outside the domain");
     }
-    if (mode == CPIM_SAC_BATCHED) {
-      SacScratch S;
-      run_singleton_batched(m, t, count, fail_out, st, S);
-    } else if (mode == CPIM_SAC_PLAIN) {
-      run_singleton_plain(m, t, count, fail_out, st);
-    } else {
-      fail(CPIM_ERR_INVALID_ARGUMENT, "unknown singleton-test mode");
-    }
+    REQUIRE(mode >= CPIM_SAC_BATCHED && mode <= CPIM_SAC_SLICED, "unknown singleton-test mode");
+    SacScratch S;
+    run_singleton_device(m, mode, t, count, fail_out, st, S);
     st->consistent = 1;
     st->passes = 1;
   });
@@ -1111,7 +1218,7 @@ int cpim_gmodel_singleton_tests(cpim_gmodel* m, const int32_t* tests, int64_t co
 int cpim_gmodel_enforce_sac(cpim_gmodel* m, int32_t mode, cpim_sac_stats* st) {
   return guarded([&] {
     REQUIRE(m && st, "null argument");
-    REQUIRE(mode == CPIM_SAC_BATCHED || mode == CPIM_SAC_PLAIN, "unknown SAC mode");
+    REQUIRE(mode >= CPIM_SAC_BATCHED && mode <= CPIM_SAC_SLICED, "unknown SAC mode");
     DeviceGuard g(m->device);
     std::memset(st, 0, sizeof(*st));
     const int L = m->current_level;
@@ -1127,8 +1234,7 @@ int cpim_gmodel_enforce_sac(cpim_gmodel* m, int32_t mode, cpim_sac_stats* st) {
       std::vector<u64> dom = read_level(m, L);
       std::vector<int2> tests = all_tests(m, dom);
       std::vector<unsigned char> fl(tests.size(), 0);
-      if (mode == CPIM_SAC_BATCHED) run_singleton_batched(m, tests.data(), (long long)tests.size(), fl.data(), st, S);
-      else run_singleton_plain(m, tests.data(), (long long)tests.size(), fl.data(), st);
+      run_singleton_device(m, mode, tests.data(), (long long)tests.size(), fl.data(), st, S);
       ++st->passes;
       long long removed = 0;
       for (size_t i = 0; i < tests.size(); ++i)

@@ -689,6 +689,263 @@ __global__ void k_singleton_tests(DevNet net, const u64* __restrict__ base_dom,
 }
 
 // ---------------------------------------------------------------------------
+// Bit-sliced singleton tests (binary networks).  A workgroup runs up to 64
+// singleton tests together: sl[v][b] is the 64-bit mask of the tests in
+// which value b of v is alive.  Revising u <- t for all of them at once,
+// value b of u keeps test k iff some value a of t with R(b, a) has bit k in
+// sl[t][a].  One support row serves 64 tests, and each value stops scanning
+// its supports as soon as every test that holds it has found one.
+//
+// Every test's bits evolve as a chaotic AC iteration of that test alone
+// (a revision is monotone and idempotent, and a revision caused by another
+// test's change is merely redundant), and a round ends with no change only
+// when every arc is consistent in every test.  So each test reaches the GAC
+// closure of its own network (unique), and test k failed iff some variable
+// holds no value with bit k.  The tests' variables are not skipped as
+// assigned: with x reduced to {a} the closure is the same either way.
+//
+// The slices live in global memory (n * dpad words per workgroup), read with
+// agent-scope loads so that no stale L1 line is ever used; the source slice
+// of the current work item is staged in wave-private LDS.
+// ---------------------------------------------------------------------------
+constexpr int kSliceArcs = 16;  // arcs of one variable per work item
+
+struct SliceCtl {
+  int item_ctr;
+  int nchanged[2];
+  int m;
+  int total;
+  int pad;
+  u64 act;  // tests not wiped out
+  unsigned long long revs;
+};
+
+__host__ __device__ inline long long slice_lds_bytes(int n, int dpad, int nwaves) {
+  const long long nvw = (n + 31) / 32;
+  long long b = 4ll * n + 4ll * (n + 1) + 8ll * nvw;
+  b = (b + 15) & ~15ll;
+  b += 8ll * nwaves * (dpad + CPIM_MAX_WORDS);
+  return b;
+}
+
+__device__ __forceinline__ void slice_mark(unsigned* nxt, SliceCtl* ctl, int v, int slot) {
+  const unsigned bit = 1u << (v & 31);
+  const unsigned old = atomicOr(&nxt[v >> 5], bit);
+  if (!(old & bit)) atomicAdd(&ctl->nchanged[slot], 1);
+}
+
+// Revise u <- t for every live test (act).  plane_u: rows of the values of u
+// over t's domain (element (w, b) = word w of the row of b); slu: slices of
+// u; stage: slices of t; atw[w]: values of t alive in some live test.
+// Returns (wave-uniform) whether some bit of u was cleared by this wave.
+__device__ __forceinline__ bool slice_revise(const u64* __restrict__ plane_u, u64* slu,
+                                             const u64* stage, const u64* atw, int W, int dpad,
+                                             u64 act) {
+  const int lane = lane_id();
+  bool changed = false;
+  for (int j0 = 0; j0 < W; j0 += 4) {
+    u64 need[4], row0[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = j0 + q;
+      need[q] = j < W ? ld_dom<true>(&slu[j * 64 + lane]) : 0ull;
+      row0[q] = j < W ? plane_u[j * 64 + lane] : 0ull;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int b = (j0 + q) * 64 + lane;
+      u64 pend = need[q] & act;  // live tests holding b that still need a support
+      for (int w = 0; w < W && pend; ++w) {
+        u64 r = (w == 0 ? row0[q] : plane_u[(long long)w * dpad + b]) & atw[w];
+        const u64* st = stage + w * 64;
+        while (r && pend) {
+          const int a1 = (int)__builtin_ctzll(r);
+          r &= r - 1;
+          u64 s = st[a1];
+          if (r) {
+            const int a2 = (int)__builtin_ctzll(r);
+            r &= r - 1;
+            s |= st[a2];
+          }
+          pend &= ~s;
+        }
+      }
+      if (pend) {
+        const u64 old = atomicAnd(&slu[b], ~pend);
+        if (old & pend) changed = true;
+      }
+    }
+  }
+  return __ballot(changed) != 0ull;
+}
+
+// Workgroup `blockIdx.x` runs test group g (tests [gfirst[g], gfirst[g+1])
+// of the list sorted by variable) in its slice region; fail[perm[i]]
+// receives the outcome of sorted test i.  Blocks are mapped to groups so
+// that each XCD takes a contiguous run of groups (same variables, shared
+// support rows in its L2).
+__global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
+                               const int2* __restrict__ tests, const int* __restrict__ gfirst,
+                               long long g0, const int* __restrict__ perm, unsigned char* fail,
+                               unsigned long long* revs_out, u64* slices) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ SliceCtl ctl;
+  const int n = net.n, W = net.W, dpad = net.dpad;
+  const int nvw = (n + 31) / 32;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  const int lane = lane_id();
+  const int wave = uniform_int(tid >> 6), nwaves = nthreads >> 6;
+  const int B = gridDim.x, bid = blockIdx.x;
+  const int xcd = bid & 7, q = B >> 3, rem = B & 7;
+  const long long g = g0 + (long long)xcd * q + (xcd < rem ? xcd : rem) + (bid >> 3);
+  const int tf = gfirst[g];
+  const int tc = uniform_int(gfirst[g + 1] - tf);
+  const u64 all = tc >= 64 ? ~0ull : ((1ull << tc) - 1ull);
+  u64* sl = slices + (long long)bid * n * dpad;
+
+  int* list = (int*)smem;
+  int* pre = list + n;
+  unsigned* cur = (unsigned*)(pre + n + 1);
+  unsigned* nxt = cur + nvw;
+  const long long head = (4ll * n + 4ll * (n + 1) + 8ll * nvw + 15) & ~15ll;
+  u64* stage = (u64*)(smem + head) + (long long)wave * (dpad + CPIM_MAX_WORDS);
+  u64* atw = stage + dpad;
+
+  // Slices of the base level: every test holds every base value.
+  for (int v = wave; v < n; v += nwaves) {
+    for (int b = lane; b < dpad; b += 64) {
+      const bool in = (base_dom[(long long)v * W + (b >> 6)] >> (b & 63)) & 1ull;
+      sl[(long long)v * dpad + b] = in ? all : 0ull;
+    }
+  }
+  for (int i = tid; i < nvw; i += nthreads) {
+    cur[i] = 0u;
+    nxt[i] = 0u;
+  }
+  if (tid == 0) {
+    ctl.act = all;
+    ctl.revs = 0;
+  }
+  __threadfence();
+  __syncthreads();
+  // Test k reduces its variable x_k to {a_k}.
+  for (int k = wave; k < tc; k += nwaves) {
+    const int2 tk = tests[tf + k];
+    for (int b = lane; b < dpad; b += 64) {
+      u64* p = &sl[(long long)tk.x * dpad + b];
+      if (b == tk.y) atomicOr(p, 1ull << k);
+      else atomicAnd(p, ~(1ull << k));
+    }
+  }
+  if (tid < tc) {
+    const int x = tests[tf + tid].x;
+    atomicOr(&cur[x >> 5], 1u << (x & 31));
+  }
+  __threadfence();
+  __syncthreads();
+
+  unsigned long long revs = 0;
+  for (int round = 0;; ++round) {
+    const int slot = round & 1;
+    if (wave == 0) {
+      int base = 0, ibase = 0;
+      for (int v0 = 0; v0 < n; v0 += 64) {
+        const int v = v0 + lane;
+        const bool f = v < n && bit_of(cur, v);
+        const u64 mask = __ballot(f);
+        const int rank = mbcnt(mask);
+        const int items = f ? (net.adj_off[v + 1] - net.adj_off[v] + kSliceArcs - 1) / kSliceArcs : 0;
+        const int inc = wave_inclusive_scan(items);
+        if (f) {
+          list[base + rank] = v;
+          pre[base + rank] = ibase + inc - items;
+        }
+        base += __popcll(mask);
+        ibase += __shfl(inc, 63, 64);
+      }
+      if (lane == 0) {
+        pre[base] = ibase;
+        ctl.m = base;
+        ctl.total = ibase;
+        ctl.item_ctr = 0;
+        ctl.nchanged[slot] = 0;
+      }
+    } else {
+      for (int i = tid - 64; i < nvw; i += nthreads - 64) nxt[i] = 0u;
+    }
+    if (nthreads == 64)
+      for (int i = lane; i < nvw; i += 64) nxt[i] = 0u;
+    __syncthreads();
+    const int total = uniform_int(ctl.total);
+    const int m = uniform_int(ctl.m);
+    const u64 act = (u64)uniform_ll((long long)ctl.act);
+    while (true) {
+      int idx = 0;
+      if (lane == 0) idx = atomicAdd(&ctl.item_ctr, 1);
+      idx = uniform_int(__shfl(idx, 0, 64));
+      if (idx >= total) break;
+      int lo = 0, hi = m - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= idx) lo = mid;
+        else hi = mid - 1;
+      }
+      const int t = uniform_int(list[lo]);
+      const int s0 = net.adj_off[t] + (idx - uniform_int(pre[lo])) * kSliceArcs;
+      const int s1 = min(s0 + kSliceArcs, net.adj_off[t + 1]);
+      const int cnt = uniform_int(s1 - s0);
+      // stage the slices of t and the values alive in some live test
+      const u64* slt = sl + (long long)t * dpad;
+      for (int i = lane; i < dpad; i += 64) {
+        const u64 v = ld_dom<true>(&slt[i]);
+        stage[i] = v;
+        const u64 alive = __ballot((v & act) != 0ull);
+        if (lane == 0) atw[i >> 6] = alive;
+      }
+      __builtin_amdgcn_wave_barrier();
+      int4 e = make_int4(0, 0, 0, 0);
+      long long off = 0;
+      if (lane < cnt) {
+        e = net.adj[s0 + lane];
+        off = net.con[e.x].off;
+      }
+      for (int i = 0; i < cnt; ++i) {
+        const int u = __builtin_amdgcn_readlane(e.y, i);
+        const int pos_t = __builtin_amdgcn_readlane(e.z, i);
+        const long long o = (long long)readlane64((u64)off, i);
+        const u64* plane_u = net.sup + o + (long long)(1 - pos_t) * W * dpad;
+        const bool ch = slice_revise(plane_u, sl + (long long)u * dpad, stage, atw, W, dpad, act);
+        ++revs;
+        if (ch && lane == 0) slice_mark(nxt, &ctl, u, slot);
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    __threadfence();
+    __syncthreads();
+    // Exact wipe-out check: a test wiped out at v only if v changed.
+    u64 alive_all = ~0ull;
+    for (int v = wave; v < n; v += nwaves) {
+      if (!uniform_int((int)bit_of(nxt, v))) continue;
+      u64 o = 0ull;
+      for (int i = lane; i < dpad; i += 64) o |= ld_dom<true>(&sl[(long long)v * dpad + i]);
+      for (int d = 32; d > 0; d >>= 1) o |= __shfl_xor(o, d, 64);
+      alive_all &= o;
+    }
+    if (lane == 0 && alive_all != ~0ull) atomicAnd(&ctl.act, alive_all);
+    __syncthreads();
+    const u64 act_now = (u64)uniform_ll((long long)ctl.act);
+    if (uniform_int(ctl.nchanged[slot]) == 0 || act_now == 0ull) break;
+    unsigned* tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+  if (lane == 0 && revs) atomicAdd(&ctl.revs, revs);
+  __syncthreads();
+  if (tid < tc) fail[perm[tf + tid]] = ((ctl.act >> tid) & 1ull) ? 0 : 1;
+  if (tid == 0 && ctl.revs) atomicAdd(revs_out, ctl.revs);
+}
+
+// ---------------------------------------------------------------------------
 // Device MAC (binary branching, dom-min over undecided variables with the
 // lowest id on ties, min value), CPU MAC::enforce semantics
 // (src/MAC.cpp:100-165).  The trail holds the domain of every depth: a

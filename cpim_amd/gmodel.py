@@ -24,6 +24,18 @@ from . import _native as N
 from .model import ModelDesc
 
 
+_SAC_MODES = {"auto": N.CPIM_SAC_BATCHED, "plain": N.CPIM_SAC_PLAIN,
+              "workgroup": N.CPIM_SAC_WORKGROUP, "sliced": N.CPIM_SAC_SLICED}
+
+
+def _sac_mode(batched: bool, mode: Optional[str]) -> int:
+    if mode is not None:
+        if mode not in _SAC_MODES:
+            raise ValueError(f"unknown SAC mode {mode!r} (one of {sorted(_SAC_MODES)})")
+        return _SAC_MODES[mode]
+    return N.CPIM_SAC_BATCHED if batched else N.CPIM_SAC_PLAIN
+
+
 @dataclass
 class GModelOptions:
     device_id: int = 0
@@ -228,20 +240,22 @@ class GModel:
         N.check(N.gpu_lib().cpim_gmodel_write_domains(self.handle, level, dom.ctypes.data_as(N.p_u64)))
 
     # -- SAC / singleton tests ---------------------------------------------------
-    def EnforceSAC(self, batched: bool = True) -> SacStats:
+    def EnforceSAC(self, batched: bool = True, mode: Optional[str] = None) -> SacStats:
+        """SAC closure.  mode: "auto" (default when batched), "sliced" (64 tests per
+        workgroup, binary networks), "workgroup" (one test per workgroup) or "plain"."""
         st = N.CSacStats()
-        mode = N.CPIM_SAC_BATCHED if batched else N.CPIM_SAC_PLAIN
-        N.check(N.gpu_lib().cpim_gmodel_enforce_sac(self.handle, mode, C.byref(st)))
+        N.check(N.gpu_lib().cpim_gmodel_enforce_sac(self.handle, _sac_mode(batched, mode),
+                                                    C.byref(st)))
         return SacStats(bool(st.consistent), st.passes, st.singleton_tests, st.removed, st.revisions,
                         st.kernel_ms)
 
-    def SingletonTests(self, tests: np.ndarray, batched: bool = True):
+    def SingletonTests(self, tests: np.ndarray, batched: bool = True, mode: Optional[str] = None):
+        """fail[i] = 1 iff GAC with D(x_i) = {a_i} wipes out (modes as in EnforceSAC)."""
         tests = np.ascontiguousarray(tests, np.int32).reshape(-1, 2)
         fail = np.zeros(len(tests), np.uint8)
         st = N.CSacStats()
-        mode = N.CPIM_SAC_BATCHED if batched else N.CPIM_SAC_PLAIN
         N.check(N.gpu_lib().cpim_gmodel_singleton_tests(
-            self.handle, tests.ctypes.data_as(N.p_i32), len(tests), mode,
+            self.handle, tests.ctypes.data_as(N.p_i32), len(tests), _sac_mode(batched, mode),
             fail.ctypes.data_as(N.p_u8), C.byref(st)))
         return fail, SacStats(bool(st.consistent), st.passes, st.singleton_tests, st.removed,
                               st.revisions, st.kernel_ms)

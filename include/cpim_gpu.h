@@ -201,8 +201,12 @@ typedef struct cpim_sac_stats {
   double kernel_ms;
 } cpim_sac_stats;
 
-#define CPIM_SAC_BATCHED 0 /* every singleton test of a pass in one persistent launch */
-#define CPIM_SAC_PLAIN 1   /* one host-driven fixpoint per singleton test */
+#define CPIM_SAC_BATCHED 0   /* every singleton test of a pass on the device: SLICED when the network
+                                allows it (binary constraints only), else WORKGROUP */
+#define CPIM_SAC_PLAIN 1     /* one host-driven fixpoint per singleton test */
+#define CPIM_SAC_WORKGROUP 2 /* one workgroup per singleton test, domain copy in LDS */
+#define CPIM_SAC_SLICED 3    /* 64 singleton tests per workgroup, bit-sliced; CPIM_ERR_UNSUPPORTED
+                                for networks with table constraints */
 
 /* SAC closure of the current level, in place.  stats->consistent = 0 on
  * wipe-out (the level then holds no meaningful domains). */

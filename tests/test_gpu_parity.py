@@ -58,11 +58,15 @@ def test_golden_root_gac(gpu, name, mode):
             assert sizes == [len(s) for s in domains_to_sets(dom, desc.dom_size)]
 
 
+SAC_MODES = ["auto", "workgroup", "sliced"]
+
+
+@pytest.mark.parametrize("mode", SAC_MODES)
 @pytest.mark.parametrize("name", golden_names())
-def test_golden_sac(gpu, name):
+def test_golden_sac(gpu, name, mode):
     desc, g = load_golden(name)
     with GModelAdapter.Build(desc) as m:
-        st = m.EnforceSAC(batched=True)
+        st = m.EnforceSAC(mode=mode)
         assert st.consistent == bool(g["sac_ok"])
         if g["sac_ok"]:
             assert (m.ReadDomains(0) == g["sac_dom"]).all()
@@ -219,10 +223,60 @@ def test_random_sac_and_singletons(gpu, case):
         want, _ = o.singleton_tests(tests, nthreads=4)
         with GModelAdapter.Build(desc) as m:
             m.EnforceGAC(False)
-            got, _ = m.SingletonTests(tests, batched=True)
-            assert (got == want).all()
+            for mode in SAC_MODES:
+                got, _ = m.SingletonTests(tests, mode=mode)
+                assert (got == want).all(), mode
             got2, _ = m.SingletonTests(tests[:64], batched=False)
             assert (got2 == want[:64]).all()
+    for mode in ("workgroup", "sliced"):
+        with GModelAdapter.Build(desc) as m:
+            st = m.EnforceSAC(mode=mode)
+            assert st.consistent == ok, mode
+            if ok:
+                assert (m.ReadDomains(0) == dom).all(), mode
+
+
+# Arbitrary test lists for the bit-sliced kernel: unsorted, repeated tests,
+# values outside the current domain (the oracle reduces D(x) to {a} all the
+# same), several variables per group, one-word to sixteen-word domains.
+SLICED = [  # (n, d, e, p, preremove, tests)
+    (40, 7, 200, 0.30, 0.2, 500), (30, 64, 150, 0.75, 0.3, 900), (20, 65, 60, 0.85, 0.0, 700),
+    (15, 1000, 30, 0.97, 0.1, 400), (60, 3, 300, 0.15, 0.0, 300),
+]
+
+
+@pytest.mark.parametrize("case", range(len(SLICED)))
+def test_sliced_arbitrary_test_lists(gpu, case):
+    n, d, e, p, pre, k = SLICED[case]
+    desc = workload.random_binary(n, d, e, p, 1700 + case, preremove=pre)
+    o = oracle.Oracle(desc)
+    rok, root, _ = o.root_ac()
+    rng = np.random.default_rng(case)
+    tests = np.stack([rng.integers(0, n, k), rng.integers(0, d, k)], 1).astype(np.int32)
+    tests[k // 2:k // 2 + 20] = tests[:20]  # repeats
+    want, _ = o.singleton_tests(tests, nthreads=4)
+    with GModelAdapter.Build(desc) as m:
+        m.EnforceGAC(False)
+        if rok:
+            assert (m.ReadDomains(0) == root).all()
+        for mode in ("sliced", "workgroup"):
+            got, st = m.SingletonTests(tests, mode=mode)
+            assert st.singleton_tests == k
+            assert (got == want).all(), mode
+        # a list shorter than one group, and a single test
+        got, _ = m.SingletonTests(tests[:5], mode="sliced")
+        assert (got == want[:5]).all()
+        got, _ = m.SingletonTests(tests[7:8], mode="sliced")
+        assert (got == want[7:8]).all()
+
+
+def test_sliced_rejects_table_networks(gpu):
+    desc = ternary(1)
+    with GModelAdapter.Build(desc, GModelOptions(skip_non_binary=False)) as m:
+        with pytest.raises(N.CpimError, match="bit-sliced"):
+            m.SingletonTests(np.array([[0, 0]], np.int32), mode="sliced")
+        # auto falls back to one test per workgroup
+        m.SingletonTests(np.array([[0, 0]], np.int32))
 
 
 @pytest.mark.parametrize("case", range(len(RANDOM)))
@@ -312,8 +366,9 @@ def test_batched_sac_full_size_sample(gpu):
     with GModelAdapter.Build(desc) as m:
         st = m.EnforceGAC_Auto()
         assert not st.inconsistent and (m.ReadDomains(0) == root).all()
-        got, sst = m.SingletonTests(sample)
-        assert (got == want).all()
+        for mode in SAC_MODES:
+            got, sst = m.SingletonTests(sample, mode=mode)
+            assert (got == want).all(), mode
         # every singleton test of the full pass, in one launch
         allf, sst = m.SingletonTests(tests)
         assert sst.singleton_tests == len(tests)

@@ -30,7 +30,7 @@ def test_no_barrier_loop_exits_through_exec_mask(engine_asm):
 
 def test_every_workgroup_kernel_is_checked(engine_asm):
     names = barrier_loops_with_exec_exit(engine_asm)
-    for k in ("k_singleton_tests", "k_mac", "k_mac_units", "k_block_gac"):
+    for k in ("k_singleton_tests", "k_sliced_tests", "k_mac", "k_mac_units", "k_block_gac"):
         assert any(k in n for n in names), k
 
 
