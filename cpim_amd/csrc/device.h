@@ -103,24 +103,37 @@ __device__ __forceinline__ u64 mix64(u64 z) {
 // ---------------------------------------------------------------------------
 // Arc revision u <- t of a binary constraint, executed by one wave.
 //
-// Keeps value b of u iff some value of t supports it.  If t has one live value
-// a, the new D(u) is D(u) AND row_t[a] (W loads).  Otherwise every live value
-// of u checks its row against D(t), word by word with early exit (lanes = 64
-// values of u per chunk).  The result mask is applied with atomic AND, the
-// exact number of values this wave removed is subtracted from size[u], and the
-// wave that brings size[u] to 0 reports the wipe-out.  Concurrent revisions of
-// the same domain are safe: domains only shrink, a stale (larger) D(t) can only
-// keep too much, and every removal re-enqueues the constraints of u.
+// Keeps value b of u iff some value of t supports it.  Two ways to get the
+// new D(u), chosen by |D(t)|:
+//  * source side, when |D(t)| * Wp <= 64 (Wp = W rounded up to a power of
+//    two): lane (k, w) loads word w of the row of the k-th live value of t,
+//    and an OR across the lanes of each word gives OR_a row_t[a] -- one
+//    round of independent loads, however the values of t are spread;
+//  * target side otherwise: lanes are 64 values of u; the word of D(t)'s
+//    first non-empty word is loaded for every live chunk of u before any is
+//    tested, and a value whose support is not in that word scans the later
+//    words (rare when D(t) is dense).
+// The result mask is applied with atomic AND, the exact number of values
+// this wave removed is subtracted from size[u], and the wave that brings
+// size[u] to 0 reports the wipe-out.  Concurrent revisions of the same
+// domain are safe: domains only shrink, a stale (larger) D(t) can only keep
+// too much, and every removal re-enqueues the constraints of u.
 // Returns the number of values removed (wave-uniform); *wiped is set when
-// this wave emptied D(u).
+// this wave emptied D(u).  `off` is the constraint's offset in sup.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int pow2_at_least(int w) {
+  int p = 1;
+  while (p < w) p <<= 1;
+  return p;
+}
+
 template <bool kCoherent = false>
-__device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* size, int c, int u,
-                                             int t, int pos_t, bool* wiped) {
+__device__ __forceinline__ int revise_binary_at(const DevNet& net, u64* dom, int* size, long long off, int u,
+                                                int t, int pos_t, bool* wiped) {
   const int W = net.W;
   const int dpad = net.dpad;
   const int lane = lane_id();
-  const u64* base = net.sup + net.con[c].off;
+  const u64* base = net.sup + off;
   const u64* plane_u = base + (long long)(1 - pos_t) * W * dpad;
   const u64* plane_t = base + (long long)pos_t * W * dpad;
   const u64 dt_l = lane < W ? ld_dom<kCoherent>(&dom[(long long)t * W + lane]) : 0ull;
@@ -132,29 +145,57 @@ __device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* s
     cnt_t += __popcll(x);
     if (first_w < 0 && x) first_w = w;
   }
+  const int Wp = pow2_at_least(W);
   u64 mask_l = 0ull;
-  if (cnt_t <= 1) {
-    if (cnt_t == 1) {
-      const int a = first_w * 64 + (int)__builtin_ctzll(readlane64(dt_l, first_w));
-      if (lane < W) mask_l = plane_t[(long long)lane * dpad + a];
-    }
-  } else {
-    for (int j = 0; j < W; ++j) {
-      const u64 duj = readlane64(du_l, j);
-      if (duj == 0ull) continue;
-      const int b = j * 64 + lane;
-      bool keep = false;
-      if ((duj >> lane) & 1ull) {
-        for (int w = 0; w < W; ++w) {
-          const u64 dtw = readlane64(dt_l, w);
-          if (plane_u[(long long)w * dpad + b] & dtw) {
-            keep = true;
-            break;
-          }
+  if (cnt_t * Wp <= 64) {
+    const int w = lane & (Wp - 1);
+    int k = lane / Wp;
+    u64 row = 0ull;
+    if (w < W && k < cnt_t) {
+      int a = -1;
+      for (int j = 0; j < W && a < 0; ++j) {
+        u64 x = readlane64(dt_l, j);
+        const int c = __popcll(x);
+        if (k < c) {
+          for (int i = 0; i < k; ++i) x &= x - 1;
+          a = j * 64 + (int)__builtin_ctzll(x);
+        } else {
+          k -= c;
         }
       }
-      const u64 mj = __ballot(keep);
-      if (lane == j) mask_l = mj;
+      row = plane_t[(long long)w * dpad + a];
+    }
+    for (int o = Wp; o < 64; o <<= 1) row |= __shfl_xor(row, o, 64);
+    mask_l = lane < W ? row : 0ull;
+  } else {
+    const u64 dt0 = readlane64(dt_l, first_w);
+    const u64* plane_u0 = plane_u + (long long)first_w * dpad;
+    for (int j0 = 0; j0 < W; j0 += 4) {
+      u64 row[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = j0 + q;
+        const bool live = j < W && readlane64(du_l, j < W ? j : 0) != 0ull;
+        row[q] = live ? plane_u0[j * 64 + lane] : 0ull;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = j0 + q;
+        if (j >= W) break;
+        const u64 duj = readlane64(du_l, j);
+        if (duj == 0ull) continue;
+        const int b = j * 64 + lane;
+        bool keep = false;
+        if ((duj >> lane) & 1ull) {
+          keep = (row[q] & dt0) != 0ull;
+          for (int w = first_w + 1; w < W && !keep; ++w) {
+            const u64 dtw = readlane64(dt_l, w);
+            if (dtw && (plane_u[(long long)w * dpad + b] & dtw)) keep = true;
+          }
+        }
+        const u64 mj = __ballot(keep);
+        if (lane == j) mask_l = mj;
+      }
     }
   }
   int removed_l = 0;
@@ -173,6 +214,12 @@ __device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* s
   }
   *wiped = wipe;
   return removed;
+}
+
+template <bool kCoherent = false>
+__device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* size, int c, int u,
+                                             int t, int pos_t, bool* wiped) {
+  return revise_binary_at<kCoherent>(net, dom, size, net.con[c].off, u, t, pos_t, wiped);
 }
 
 // ---------------------------------------------------------------------------

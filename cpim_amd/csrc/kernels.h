@@ -440,7 +440,11 @@ __device__ __forceinline__ void wg_mark(unsigned* nxt, WgCtl* ctl, int v, int sl
 // AC fixpoint from the seed in mem.cur.  both_dirs_round0: in the first round
 // also revise t <- u for every arc of a seed variable t (constraint seeding,
 // as GModel::EnforceGAC(assigned_var) does).  Returns false on wipe-out.
-// Must be called by every thread of the workgroup.
+// Must be called by every thread of the workgroup.  Work items are runs of
+// up to kWgArcs arcs of one changed variable: the lanes of a wave fetch the
+// arcs' metadata together, then the wave revises them one by one.
+constexpr int kWgArcs = 8;
+
 __device__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_dirs_round0) {
   const int n = net.n;
   const int nvw = (n + 31) / 32;
@@ -456,25 +460,25 @@ __device__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_
   for (int round = 0;; ++round) {
     const int slot = round & 1;
     if (wave == 0) {
-      int base = 0, dbase = 0;
+      int base = 0, ibase = 0;
       for (int v0 = 0; v0 < n; v0 += 64) {
         const int v = v0 + lane;
         const bool f = v < n && bit_of(cur, v);
         const u64 mask = __ballot(f);
         const int rank = mbcnt(mask);
-        const int deg = f ? net.adj_off[v + 1] - net.adj_off[v] : 0;
-        const int inc = wave_inclusive_scan(deg);
+        const int items = f ? (net.adj_off[v + 1] - net.adj_off[v] + kWgArcs - 1) / kWgArcs : 0;
+        const int inc = wave_inclusive_scan(items);
         if (f) {
           mem.list[base + rank] = v;
-          mem.pre[base + rank] = dbase + inc - deg;
+          mem.pre[base + rank] = ibase + inc - items;
         }
         base += __popcll(mask);
-        dbase += __shfl(inc, 63, 64);
+        ibase += __shfl(inc, 63, 64);
       }
       if (lane == 0) {
-        mem.pre[base] = dbase;
+        mem.pre[base] = ibase;
         ctl->m = base;
-        ctl->total = dbase;
+        ctl->total = ibase;
         ctl->arc_ctr = 0;
         ctl->nchanged[slot] = 0;
       }
@@ -499,32 +503,45 @@ __device__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_
         if (mem.pre[mid] <= idx) lo = mid;
         else hi = mid - 1;
       }
-      const int t = mem.list[lo];
-      const int4 e = net.adj[net.adj_off[t] + (idx - mem.pre[lo])];
-      const int c = e.x;
-      bool wiped = false;
-      if (net.con[c].kind == 0) {
-        const int u = e.y;
-        if (!bit_of(mem.asg, u)) {
-          const int r = revise_binary(net, mem.dom, mem.size, c, u, t, e.z, &wiped);
+      const int t = uniform_int(mem.list[lo]);
+      const int s0 = net.adj_off[t] + (idx - uniform_int(mem.pre[lo])) * kWgArcs;
+      const int cnt = uniform_int(min(s0 + kWgArcs, net.adj_off[t + 1]) - s0);
+      int4 e = make_int4(0, 0, 0, 0);
+      int kind = 0;
+      long long off = 0;
+      if (lane < cnt) {
+        e = net.adj[s0 + lane];
+        kind = net.con[e.x].kind;
+        off = net.con[e.x].off;
+      }
+      for (int i = 0; i < cnt; ++i) {
+        const int c = __builtin_amdgcn_readlane(e.x, i);
+        bool wiped = false;
+        if (__builtin_amdgcn_readlane(kind, i) == 0) {
+          const int u = __builtin_amdgcn_readlane(e.y, i);
+          const int pos_t = __builtin_amdgcn_readlane(e.z, i);
+          const long long o = (long long)readlane64((u64)off, i);
+          if (!bit_of(mem.asg, u)) {
+            const int r = revise_binary_at(net, mem.dom, mem.size, o, u, t, pos_t, &wiped);
+            ++revs;
+            if (r && lane == 0) wg_mark(nxt, ctl, u, slot);
+            if (wiped && lane == 0) ctl->wipe = 1;
+          }
+          if (both && !bit_of(mem.asg, t)) {
+            const int r = revise_binary_at(net, mem.dom, mem.size, o, t, u, 1 - pos_t, &wiped);
+            ++revs;
+            if (r && lane == 0) wg_mark(nxt, ctl, t, slot);
+            if (wiped && lane == 0) ctl->wipe = 1;
+          }
+        } else {
+          revise_table(net, mem.dom, mem.size, c, both ? -1 : t, mem.asg,
+                       [&](int v, int) {
+                         if (lane == 0) wg_mark(nxt, ctl, v, slot);
+                       },
+                       &wiped);
           ++revs;
-          if (r && lane == 0) wg_mark(nxt, ctl, u, slot);
           if (wiped && lane == 0) ctl->wipe = 1;
         }
-        if (both && !bit_of(mem.asg, t)) {
-          const int r = revise_binary(net, mem.dom, mem.size, c, t, u, 1 - e.z, &wiped);
-          ++revs;
-          if (r && lane == 0) wg_mark(nxt, ctl, t, slot);
-          if (wiped && lane == 0) ctl->wipe = 1;
-        }
-      } else {
-        revise_table(net, mem.dom, mem.size, c, both ? -1 : t, mem.asg,
-                     [&](int v, int) {
-                       if (lane == 0) wg_mark(nxt, ctl, v, slot);
-                     },
-                     &wiped);
-        ++revs;
-        if (wiped && lane == 0) ctl->wipe = 1;
       }
     }
     __syncthreads();
