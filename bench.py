@@ -42,6 +42,8 @@ def parse():
                     help="singleton tests timed on the CPU oracle (about 10-15 s)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--sac-mode", default="sliced", choices=["sliced", "workgroup"],
+                    help="pass kernel: 64 bit-sliced tests per workgroup, or one test per workgroup")
     ap.add_argument("--skip-extras", action="store_true",
                     help="skip the single-fixpoint, uneven-batch and serving measurements")
     ap.add_argument("--batch-instances", type=int, default=512,
@@ -55,14 +57,19 @@ def parse():
     return ap.parse_args()
 
 
-PROFILE = os.path.join(REPO, "profiles", "batched_sac_singleton_tests.txt")
+# committed rocprofv3 summaries of the pass kernel, per SAC mode
+PROFILES = {"sliced": os.path.join(REPO, "profiles", "k_sliced_tests_batched_sac.txt"),
+            "workgroup": os.path.join(REPO, "profiles", "k_singleton_tests_batched_sac.txt")}
 
 
-def profiled_traffic() -> float | None:
-    """HBM bytes per k_singleton_tests launch on this workload, from the committed
+def profiled_traffic(mode: str) -> float | None:
+    """HBM bytes per launch of the pass kernel on this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel calibration)."""
+    path = PROFILES.get(mode)
+    if path is None:
+        return None
     try:
-        with open(PROFILE) as f:
+        with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
@@ -220,11 +227,11 @@ def main():
     bytes_per_pass = float(np.sum(8.0 * W * (4.0 * deg[tests[:, 0]] + n)))
 
     for _ in range(args.warmup):
-        m.SingletonTests(tests)
+        m.SingletonTests(tests, mode=args.sac_mode)
     kernel_ms = []
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        fail, st = m.SingletonTests(tests)
+        fail, st = m.SingletonTests(tests, mode=args.sac_mode)
         kernel_ms.append(st.kernel_ms)
     elapsed = time.perf_counter() - t_start
     ms_per_step = 1000.0 * elapsed / args.steps
@@ -278,6 +285,7 @@ def main():
             "workload": "batched SAC1 singleton tests, random binary CSP n=500 d=256 e=20000 "
                         "tightness 0.5, one SAC pass per step",
             "singleton_tests_per_step": ntests,
+            "sac_kernel": "k_sliced_tests" if args.sac_mode == "sliced" else "k_singleton_tests",
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -293,6 +301,7 @@ def main():
         "speedup_vs_plain": (value / plain) if plain else None,
         "kernel_ms_per_step": avg_kernel_ms,
         "stream_copy_gbps": copy_bandwidth(1024, 5),
+        # sliced: arc revisions of a 64-test group count once
         "revisions_per_s": st.revisions / (avg_kernel_ms * 1e-3),
     }
     if not args.skip_extras:
@@ -304,7 +313,7 @@ def main():
             except Exception as exc:  # reported in the line, never hidden
                 log(f"{key} failed: {exc!r}")
                 out[key] = {"error": repr(exc)}
-    traffic = profiled_traffic()
+    traffic = profiled_traffic(args.sac_mode)
     out["roofline"]["traffic"] = traffic
     if traffic:
         out["measured_dram_gbps"] = traffic / (avg_kernel_ms * 1e-3) / 1e9
