@@ -5,6 +5,7 @@
 #include <chrono>
 #include <climits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -379,6 +380,27 @@ long long now_ns() {
 
 }  // namespace
 
+// Device buffers of the singleton-test paths, kept by the model between
+// calls (grown on demand) so that a SAC pass allocates nothing once warm.
+struct SacScratch {
+  DevAlloc tests, fails, revs;
+  long long cap = 0;
+  DevAlloc stests, perm, gfirst;  // bit-sliced path: sorted tests, their order, group starts
+  long long scap = 0, gcap = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  SacScratch() = default;
+  SacScratch(const SacScratch&) = delete;
+  SacScratch& operator=(const SacScratch&) = delete;
+  ~SacScratch() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  void events() {
+    for (hipEvent_t& e : ev)
+      if (!e) HIPC(hipEventCreate(&e));
+  }
+};
+
 // ---------------------------------------------------------------------------
 // GModel
 // ---------------------------------------------------------------------------
@@ -401,6 +423,7 @@ struct cpim_gmodel {
   int nfw = 0;
   std::vector<int2> assigned;  // d_assigned mirror: (var, value) per level
   DevProps props;
+  SacScratch sac;
 
   u64* level_dom(int level) const { return dom.as<u64>() + (long long)level * dm.n * dm.W; }
   int* level_size(int level) const { return size.as<int>() + (long long)level * dm.n; }
@@ -961,13 +984,6 @@ int cpim_gmodel_write_domains(cpim_gmodel* m, int32_t level, const uint64_t* in)
 // ---------------------------------------------------------------------------
 namespace {
 
-struct SacScratch {
-  DevAlloc tests, fail, revs;
-  long long cap = 0;
-  DevAlloc stests, perm, gfirst;  // bit-sliced path: sorted tests, their order, group starts
-  long long scap = 0, gcap = 0;
-};
-
 // Launch k_singleton_tests over `count` tests: one workgroup per test.  With
 // domains in LDS the whole list is one launch; with global scratch the list
 // is cut into launches of kScratchBlocks workgroups.
@@ -978,7 +994,7 @@ void run_singleton_batched(cpim_gmodel* m, const int2* tests_host, long long cou
   if (count == 0) return;
   if (count > S.cap) {
     S.tests = DevAlloc(sizeof(int2) * count);
-    S.fail = DevAlloc(count);
+    S.fails = DevAlloc(count);
     S.cap = count;
   }
   if (!S.revs.p) S.revs = DevAlloc(sizeof(unsigned long long));
@@ -988,16 +1004,14 @@ void run_singleton_batched(cpim_gmodel* m, const int2* tests_host, long long cou
   DevNet net = m->dm.net();
   const long long bytes = wg_mem_bytes(m->dm.n, m->dm.W);
   const int threads = 256;
-  hipEvent_t e0, e1;
-  HIPC(hipEventCreate(&e0));
-  HIPC(hipEventCreate(&e1));
-  HIPC(hipEventRecord(e0, m->stream));
+  S.events();
+  HIPC(hipEventRecord(S.ev[0], m->stream));
   if (bytes <= kLdsBudget) {
     for (long long first = 0; first < count; first += INT_MAX) {
       const long long blocks = std::min<long long>(count - first, INT_MAX);
       hipLaunchKernelGGL(k_singleton_tests<true>, dim3((unsigned)blocks), dim3(threads), (size_t)bytes, m->stream,
                          net, m->level_dom(L), m->level_size(L), S.tests.as<int2>(), count, first,
-                         S.fail.as<unsigned char>(), S.revs.as<unsigned long long>(), (char*)nullptr, 0ll);
+                         S.fails.as<unsigned char>(), S.revs.as<unsigned long long>(), (char*)nullptr, 0ll);
       HIPC(hipGetLastError());
     }
   } else {
@@ -1006,19 +1020,17 @@ void run_singleton_batched(cpim_gmodel* m, const int2* tests_host, long long cou
       const long long blocks = std::min<long long>(count - first, kScratchBlocks);
       hipLaunchKernelGGL(k_singleton_tests<false>, dim3((unsigned)blocks), dim3(threads), 0, m->stream, net,
                          m->level_dom(L), m->level_size(L), S.tests.as<int2>(), count, first,
-                         S.fail.as<unsigned char>(), S.revs.as<unsigned long long>(), scratch, bytes);
+                         S.fails.as<unsigned char>(), S.revs.as<unsigned long long>(), scratch, bytes);
       HIPC(hipGetLastError());
     }
   }
-  HIPC(hipEventRecord(e1, m->stream));
-  HIPC(hipMemcpyAsync(fail_host, S.fail.p, count, hipMemcpyDeviceToHost, m->stream));
+  HIPC(hipEventRecord(S.ev[1], m->stream));
+  HIPC(hipMemcpyAsync(fail_host, S.fails.p, count, hipMemcpyDeviceToHost, m->stream));
   unsigned long long revs = 0;
   HIPC(hipMemcpyAsync(&revs, S.revs.p, sizeof(revs), hipMemcpyDeviceToHost, m->stream));
   m->sync();
   float ms = 0.f;
-  HIPC(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  HIPC(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
   st->kernel_ms += ms;
   st->revisions += (int64_t)revs;
   st->singleton_tests += count;
@@ -1028,39 +1040,59 @@ void run_singleton_batched(cpim_gmodel* m, const int2* tests_host, long long cou
 // 64 (a variable with more values is split into balanced groups; small
 // variables share a group), one workgroup per group.  The slices of the
 // resident groups take n * dpad words each, within kSliceBudgetBytes.
-constexpr int kSliceThreads = 256;
+constexpr int kSliceThreads = 512;
+
+// Workgroup size of the bit-sliced kernel: 512 threads (best of 128..1024 on
+// the 500-variable, 256-value SAC pass), or CPIM_SLICE_THREADS
+// (64..1024, a multiple of 64) for tuning.
+int slice_threads() {
+  static const int t = [] {
+    const char* e = std::getenv("CPIM_SLICE_THREADS");
+    const int v = e ? std::atoi(e) : kSliceThreads;
+    return (v >= 64 && v <= 1024 && v % 64 == 0) ? v : kSliceThreads;
+  }();
+  return t;
+}
 constexpr long long kSliceBudgetBytes = 4ll << 30;
 
 bool sliced_eligible(const cpim_gmodel* m) {
   return (int)m->im.bin_cons.size() == m->im.num_active &&
-         slice_lds_bytes(m->dm.n, m->dm.dpad, kSliceThreads / 64) <= kLdsBudget;
+         slice_lds_bytes(m->dm.n, m->dm.dpad, slice_threads() / 64) <= kLdsBudget;
 }
 
 void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long count, unsigned char* fail_host,
                           cpim_sac_stats* st, SacScratch& S) {
   if (count == 0) return;
   REQUIRE(count <= INT_MAX, "too many singleton tests for one call");
-  std::vector<int> order((size_t)count);
-  for (long long i = 0; i < count; ++i) order[(size_t)i] = (int)i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return tests_host[a].x < tests_host[b].x; });
-  std::vector<int2> sorted((size_t)count);
-  for (long long i = 0; i < count; ++i) sorted[(size_t)i] = tests_host[order[(size_t)i]];
+  // Lists sorted by variable (every SAC pass) are used as they are.
+  bool sorted_in = true;
+  for (long long i = 1; i < count && sorted_in; ++i) sorted_in = tests_host[i - 1].x <= tests_host[i].x;
+  std::vector<int> order;
+  std::vector<int2> sorted_buf;
+  const int2* sorted = tests_host;
+  if (!sorted_in) {
+    order.resize((size_t)count);
+    for (long long i = 0; i < count; ++i) order[(size_t)i] = (int)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return tests_host[a].x < tests_host[b].x; });
+    sorted_buf.resize((size_t)count);
+    for (long long i = 0; i < count; ++i) sorted_buf[(size_t)i] = tests_host[order[(size_t)i]];
+    sorted = sorted_buf.data();
+  }
   std::vector<int> gfirst;
+  gfirst.reserve((size_t)(count / 32 + 2));
   int open = -1;  // start of the group being filled, -1 if none
   for (long long i = 0; i < count;) {
     long long j = i;
-    while (j < count && sorted[(size_t)j].x == sorted[(size_t)i].x) ++j;
+    while (j < count && sorted[j].x == sorted[i].x) ++j;
     const long long len = j - i;
     if (len > 64) {
-      if (open >= 0) open = -1;
+      open = -1;
       const long long k = (len + 63) / 64;
       for (long long p = 0; p < k; ++p) gfirst.push_back((int)(i + len * p / k));
-    } else {
-      if (open < 0 || i + len - open > 64) {
-        gfirst.push_back((int)i);
-        open = (int)i;
-      }
+    } else if (open < 0 || i + len - open > 64) {
+      gfirst.push_back((int)i);
+      open = (int)i;
     }
     i = j;
   }
@@ -1072,7 +1104,7 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
     S.scap = count;
   }
   if (count > S.cap) {
-    S.fail = DevAlloc(count);
+    S.fails = DevAlloc(count);
     S.tests = DevAlloc(sizeof(int2) * count);
     S.cap = count;
   }
@@ -1081,8 +1113,10 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
     S.gcap = groups + 1;
   }
   if (!S.revs.p) S.revs = DevAlloc(sizeof(unsigned long long));
-  HIPC(hipMemcpyAsync(S.stests.p, sorted.data(), sizeof(int2) * count, hipMemcpyHostToDevice, m->stream));
-  HIPC(hipMemcpyAsync(S.perm.p, order.data(), sizeof(int) * count, hipMemcpyHostToDevice, m->stream));
+  S.events();
+  HIPC(hipMemcpyAsync(S.stests.p, sorted, sizeof(int2) * count, hipMemcpyHostToDevice, m->stream));
+  if (!sorted_in)
+    HIPC(hipMemcpyAsync(S.perm.p, order.data(), sizeof(int) * count, hipMemcpyHostToDevice, m->stream));
   HIPC(hipMemcpyAsync(S.gfirst.p, gfirst.data(), sizeof(int) * (groups + 1), hipMemcpyHostToDevice, m->stream));
   HIPC(hipMemsetAsync(S.revs.p, 0, sizeof(unsigned long long), m->stream));
   const long long slice_bytes = 8ll * m->dm.n * m->dm.dpad;
@@ -1090,27 +1124,23 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
   u64* slices = reinterpret_cast<u64*>(m->get_scratch(slice_bytes * per_launch));
   const int L = m->current_level;
   DevNet net = m->dm.net();
-  const size_t lds = (size_t)slice_lds_bytes(m->dm.n, m->dm.dpad, kSliceThreads / 64);
-  hipEvent_t e0, e1;
-  HIPC(hipEventCreate(&e0));
-  HIPC(hipEventCreate(&e1));
-  HIPC(hipEventRecord(e0, m->stream));
+  const size_t lds = (size_t)slice_lds_bytes(m->dm.n, m->dm.dpad, slice_threads() / 64);
+  const int* perm = sorted_in ? nullptr : S.perm.as<int>();
+  HIPC(hipEventRecord(S.ev[0], m->stream));
   for (long long g0 = 0; g0 < groups; g0 += per_launch) {
     const long long blocks = std::min(per_launch, groups - g0);
-    hipLaunchKernelGGL(k_sliced_tests, dim3((unsigned)blocks), dim3(kSliceThreads), lds, m->stream, net,
-                       m->level_dom(L), S.stests.as<int2>(), S.gfirst.as<int>(), g0, S.perm.as<int>(),
-                       S.fail.as<unsigned char>(), S.revs.as<unsigned long long>(), slices);
+    hipLaunchKernelGGL(k_sliced_tests, dim3((unsigned)blocks), dim3(slice_threads()), lds, m->stream, net,
+                       m->level_dom(L), S.stests.as<int2>(), S.gfirst.as<int>(), g0, perm,
+                       S.fails.as<unsigned char>(), S.revs.as<unsigned long long>(), slices);
     HIPC(hipGetLastError());
   }
-  HIPC(hipEventRecord(e1, m->stream));
-  HIPC(hipMemcpyAsync(fail_host, S.fail.p, count, hipMemcpyDeviceToHost, m->stream));
+  HIPC(hipEventRecord(S.ev[1], m->stream));
+  HIPC(hipMemcpyAsync(fail_host, S.fails.p, count, hipMemcpyDeviceToHost, m->stream));
   unsigned long long revs = 0;
   HIPC(hipMemcpyAsync(&revs, S.revs.p, sizeof(revs), hipMemcpyDeviceToHost, m->stream));
   m->sync();
   float ms = 0.f;
-  HIPC(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  HIPC(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
   st->kernel_ms += ms;
   st->revisions += (int64_t)revs;
   st->singleton_tests += count;
@@ -1204,7 +1234,7 @@ int cpim_gmodel_singleton_tests(cpim_gmodel* m, const int32_t* tests, int64_t co
       REQUIRE(t[i].y >= 0 && t[i].y < m->im.cap[t[i].x], "test value outside the domain");
     }
     REQUIRE(mode >= CPIM_SAC_BATCHED && mode <= CPIM_SAC_SLICED, "unknown singleton-test mode");
-    SacScratch S;
+    SacScratch& S = m->sac;
     run_singleton_device(m, mode, t, count, fail_out, st, S);
     st->consistent = 1;
     st->passes = 1;
@@ -1229,7 +1259,7 @@ int cpim_gmodel_enforce_sac(cpim_gmodel* m, int32_t mode, cpim_sac_stats* st) {
       st->consistent = 0;
       return;
     }
-    SacScratch S;
+    SacScratch& S = m->sac;
     while (true) {
       std::vector<u64> dom = read_level(m, L);
       std::vector<int2> tests = all_tests(m, dom);

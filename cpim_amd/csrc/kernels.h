@@ -727,6 +727,12 @@ __global__ void k_singleton_tests(DevNet net, const u64* __restrict__ base_dom,
 // ---------------------------------------------------------------------------
 constexpr int kSliceArcs = 16;  // arcs of one variable per work item
 
+// Staged source slices: word w of t's domain occupies kStageWord entries,
+// the masks of values 64w..64w+31 at [0, 32), a zero entry at 32, values
+// 64w+32..64w+63 at [33, 65) and a zero entry at 65.  A half-word scan reads
+// entry ctz(bits), and ctz of an exhausted (zero) half is 32: the zero entry.
+constexpr int kStageWord = 66;
+
 struct SliceCtl {
   int item_ctr;
   int nchanged[2];
@@ -741,7 +747,7 @@ __host__ __device__ inline long long slice_lds_bytes(int n, int dpad, int nwaves
   const long long nvw = (n + 31) / 32;
   long long b = 4ll * n + 4ll * (n + 1) + 8ll * nvw;
   b = (b + 15) & ~15ll;
-  b += 8ll * nwaves * (dpad + CPIM_MAX_WORDS);
+  b += 8ll * nwaves * (kStageWord * (dpad / 64) + CPIM_MAX_WORDS);
   return b;
 }
 
@@ -751,10 +757,33 @@ __device__ __forceinline__ void slice_mark(unsigned* nxt, SliceCtl* ctl, int v, 
   if (!(old & bit)) atomicAdd(&ctl->nchanged[slot], 1);
 }
 
+
+__device__ __forceinline__ int stage_pos(int v) {
+  const int b = v & 63;
+  return (v >> 6) * kStageWord + b + (b >> 5);
+}
+
+// pend &= ~(st[a] for every set bit a of bits), four bits per step.
+__device__ __forceinline__ u64 clear_supported(u64 pend, unsigned bits, const u64* st) {
+  while (bits && pend) {
+    const unsigned a0 = (unsigned)__builtin_ctzg(bits, 32);
+    bits &= bits - 1u;
+    const unsigned a1 = (unsigned)__builtin_ctzg(bits, 32);
+    bits &= bits - 1u;
+    const unsigned a2 = (unsigned)__builtin_ctzg(bits, 32);
+    bits &= bits - 1u;
+    const unsigned a3 = (unsigned)__builtin_ctzg(bits, 32);
+    bits &= bits - 1u;
+    pend &= ~((st[a0] | st[a1]) | (st[a2] | st[a3]));
+  }
+  return pend;
+}
+
 // Revise u <- t for every live test (act).  plane_u: rows of the values of u
 // over t's domain (element (w, b) = word w of the row of b); slu: slices of
-// u; stage: slices of t; atw[w]: values of t alive in some live test.
-// Returns (wave-uniform) whether some bit of u was cleared by this wave.
+// u; stage: slices of t in the kStageWord layout; atw[w]: values of t alive
+// in some live test.  Returns (wave-uniform) whether some bit of u was cleared
+// by this wave.
 __device__ __forceinline__ bool slice_revise(const u64* __restrict__ plane_u, u64* slu,
                                              const u64* stage, const u64* atw, int W, int dpad,
                                              u64 act) {
@@ -773,19 +802,10 @@ __device__ __forceinline__ bool slice_revise(const u64* __restrict__ plane_u, u6
       const int b = (j0 + q) * 64 + lane;
       u64 pend = need[q] & act;  // live tests holding b that still need a support
       for (int w = 0; w < W && pend; ++w) {
-        u64 r = (w == 0 ? row0[q] : plane_u[(long long)w * dpad + b]) & atw[w];
-        const u64* st = stage + w * 64;
-        while (r && pend) {
-          const int a1 = (int)__builtin_ctzll(r);
-          r &= r - 1;
-          u64 s = st[a1];
-          if (r) {
-            const int a2 = (int)__builtin_ctzll(r);
-            r &= r - 1;
-            s |= st[a2];
-          }
-          pend &= ~s;
-        }
+        const u64 r = (w == 0 ? row0[q] : plane_u[(long long)w * dpad + b]) & atw[w];
+        const u64* st = stage + w * kStageWord;
+        pend = clear_supported(pend, (unsigned)r, st);
+        pend = clear_supported(pend, (unsigned)(r >> 32), st + 33);
       }
       if (pend) {
         const u64 old = atomicAnd(&slu[b], ~pend);
@@ -798,7 +818,7 @@ __device__ __forceinline__ bool slice_revise(const u64* __restrict__ plane_u, u6
 
 // Workgroup `blockIdx.x` runs test group g (tests [gfirst[g], gfirst[g+1])
 // of the list sorted by variable) in its slice region; fail[perm[i]]
-// receives the outcome of sorted test i.  Blocks are mapped to groups so
+// (fail[i] when perm is null) receives the outcome of sorted test i.  Blocks are mapped to groups so
 // that each XCD takes a contiguous run of groups (same variables, shared
 // support rows in its L2).
 __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
@@ -825,8 +845,9 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
   unsigned* cur = (unsigned*)(pre + n + 1);
   unsigned* nxt = cur + nvw;
   const long long head = (4ll * n + 4ll * (n + 1) + 8ll * nvw + 15) & ~15ll;
-  u64* stage = (u64*)(smem + head) + (long long)wave * (dpad + CPIM_MAX_WORDS);
-  u64* atw = stage + dpad;
+  u64* stage = (u64*)(smem + head) + (long long)wave * (kStageWord * W + CPIM_MAX_WORDS);
+  u64* atw = stage + kStageWord * W;
+  if (lane < 2 * W) stage[(lane >> 1) * kStageWord + 32 + 33 * (lane & 1)] = 0ull;  // zero entries
 
   // Slices of the base level: every test holds every base value.
   for (int v = wave; v < n; v += nwaves) {
@@ -915,7 +936,7 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
       const u64* slt = sl + (long long)t * dpad;
       for (int i = lane; i < dpad; i += 64) {
         const u64 v = ld_dom<true>(&slt[i]);
-        stage[i] = v;
+        stage[stage_pos(i)] = v;
         const u64 alive = __ballot((v & act) != 0ull);
         if (lane == 0) atw[i >> 6] = alive;
       }
@@ -958,7 +979,7 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
   }
   if (lane == 0 && revs) atomicAdd(&ctl.revs, revs);
   __syncthreads();
-  if (tid < tc) fail[perm[tf + tid]] = ((ctl.act >> tid) & 1ull) ? 0 : 1;
+  if (tid < tc) fail[perm ? perm[tf + tid] : tf + tid] = ((ctl.act >> tid) & 1ull) ? 0 : 1;
   if (tid == 0 && ctl.revs) atomicAdd(revs_out, ctl.revs);
 }
 
