@@ -1055,9 +1055,16 @@ int slice_threads() {
 }
 constexpr long long kSliceBudgetBytes = 4ll << 30;
 
+// Threads per workgroup for this network: the configured size, halved while
+// the per-wave staging areas exceed the LDS budget (0 if even one wave does not fit).
+int slice_threads_for(const cpim_gmodel* m) {
+  int t = slice_threads();
+  while (t > 64 && slice_lds_bytes(m->dm.n, m->dm.dpad, t / 64) > kLdsBudget) t /= 2;
+  return slice_lds_bytes(m->dm.n, m->dm.dpad, t / 64) <= kLdsBudget ? t : 0;
+}
+
 bool sliced_eligible(const cpim_gmodel* m) {
-  return (int)m->im.bin_cons.size() == m->im.num_active &&
-         slice_lds_bytes(m->dm.n, m->dm.dpad, slice_threads() / 64) <= kLdsBudget;
+  return (int)m->im.bin_cons.size() == m->im.num_active && slice_threads_for(m) > 0;
 }
 
 void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long count, unsigned char* fail_host,
@@ -1124,12 +1131,13 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
   u64* slices = reinterpret_cast<u64*>(m->get_scratch(slice_bytes * per_launch));
   const int L = m->current_level;
   DevNet net = m->dm.net();
-  const size_t lds = (size_t)slice_lds_bytes(m->dm.n, m->dm.dpad, slice_threads() / 64);
+  const int threads = slice_threads_for(m);
+  const size_t lds = (size_t)slice_lds_bytes(m->dm.n, m->dm.dpad, threads / 64);
   const int* perm = sorted_in ? nullptr : S.perm.as<int>();
   HIPC(hipEventRecord(S.ev[0], m->stream));
   for (long long g0 = 0; g0 < groups; g0 += per_launch) {
     const long long blocks = std::min(per_launch, groups - g0);
-    hipLaunchKernelGGL(k_sliced_tests, dim3((unsigned)blocks), dim3(slice_threads()), lds, m->stream, net,
+    hipLaunchKernelGGL(k_sliced_tests, dim3((unsigned)blocks), dim3(threads), lds, m->stream, net,
                        m->level_dom(L), S.stests.as<int2>(), S.gfirst.as<int>(), g0, perm,
                        S.fails.as<unsigned char>(), S.revs.as<unsigned long long>(), slices);
     HIPC(hipGetLastError());

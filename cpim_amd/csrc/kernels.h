@@ -725,12 +725,12 @@ __global__ void k_singleton_tests(DevNet net, const u64* __restrict__ base_dom,
 // agent-scope loads so that no stale L1 line is ever used; the source slice
 // of the current work item is staged in wave-private LDS.
 // ---------------------------------------------------------------------------
-constexpr int kSliceArcs = 16;  // arcs of one variable per work item
+constexpr int kSliceArcs = 16;  // most arcs of one variable per work item
 
-// Staged source slices: word w of t's domain occupies kStageWord entries,
-// the masks of values 64w..64w+31 at [0, 32), a zero entry at 32, values
-// 64w+32..64w+63 at [33, 65) and a zero entry at 65.  A half-word scan reads
-// entry ctz(bits), and ctz of an exhausted (zero) half is 32: the zero entry.
+// Staged source slices: word w of t's domain occupies kStageWord entries: a
+// zero entry, the masks of values 64w..64w+31, a zero entry, the masks of
+// values 64w+32..64w+63.  A half-word scan reads entry ffbl(bits) of its
+// half, and v_ffbl_b32 of an exhausted (zero) half is -1: the zero entry.
 constexpr int kStageWord = 66;
 
 struct SliceCtl {
@@ -738,7 +738,7 @@ struct SliceCtl {
   int nchanged[2];
   int m;
   int total;
-  int pad;
+  int k;  // arcs per work item this round
   u64 act;  // tests not wiped out
   unsigned long long revs;
 };
@@ -760,19 +760,27 @@ __device__ __forceinline__ void slice_mark(unsigned* nxt, SliceCtl* ctl, int v, 
 
 __device__ __forceinline__ int stage_pos(int v) {
   const int b = v & 63;
-  return (v >> 6) * kStageWord + b + (b >> 5);
+  return (v >> 6) * kStageWord + 1 + b + (b >> 5);
 }
 
-// pend &= ~(st[a] for every set bit a of bits), four bits per step.
+// Lowest set bit of x, -1 for 0 (the hardware result of v_ffbl_b32).
+__device__ __forceinline__ int ffbl_raw(unsigned x) {
+  int r;
+  asm("v_ffbl_b32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+// pend &= ~(st[a] for every set bit a of bits), four bits per step; st[-1]
+// is the zero entry.
 __device__ __forceinline__ u64 clear_supported(u64 pend, unsigned bits, const u64* st) {
   while (bits && pend) {
-    const unsigned a0 = (unsigned)__builtin_ctzg(bits, 32);
+    const int a0 = ffbl_raw(bits);
     bits &= bits - 1u;
-    const unsigned a1 = (unsigned)__builtin_ctzg(bits, 32);
+    const int a1 = ffbl_raw(bits);
     bits &= bits - 1u;
-    const unsigned a2 = (unsigned)__builtin_ctzg(bits, 32);
+    const int a2 = ffbl_raw(bits);
     bits &= bits - 1u;
-    const unsigned a3 = (unsigned)__builtin_ctzg(bits, 32);
+    const int a3 = ffbl_raw(bits);
     bits &= bits - 1u;
     pend &= ~((st[a0] | st[a1]) | (st[a2] | st[a3]));
   }
@@ -801,11 +809,16 @@ __device__ __forceinline__ bool slice_revise(const u64* __restrict__ plane_u, u6
     for (int q = 0; q < 4; ++q) {
       const int b = (j0 + q) * 64 + lane;
       u64 pend = need[q] & act;  // live tests holding b that still need a support
-      for (int w = 0; w < W && pend; ++w) {
-        const u64 r = (w == 0 ? row0[q] : plane_u[(long long)w * dpad + b]) & atw[w];
+      if (pend) {
+        const u64 r = row0[q] & atw[0];
+        pend = clear_supported(pend, (unsigned)r, stage + 1);
+        pend = clear_supported(pend, (unsigned)(r >> 32), stage + 34);
+      }
+      for (int w = 1; w < W && pend; ++w) {
+        const u64 r = plane_u[(long long)w * dpad + b] & atw[w];
         const u64* st = stage + w * kStageWord;
-        pend = clear_supported(pend, (unsigned)r, st);
-        pend = clear_supported(pend, (unsigned)(r >> 32), st + 33);
+        pend = clear_supported(pend, (unsigned)r, st + 1);
+        pend = clear_supported(pend, (unsigned)(r >> 32), st + 34);
       }
       if (pend) {
         const u64 old = atomicAnd(&slu[b], ~pend);
@@ -847,7 +860,7 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
   const long long head = (4ll * n + 4ll * (n + 1) + 8ll * nvw + 15) & ~15ll;
   u64* stage = (u64*)(smem + head) + (long long)wave * (kStageWord * W + CPIM_MAX_WORDS);
   u64* atw = stage + kStageWord * W;
-  if (lane < 2 * W) stage[(lane >> 1) * kStageWord + 32 + 33 * (lane & 1)] = 0ull;  // zero entries
+  if (lane < 2 * W) stage[(lane >> 1) * kStageWord + 33 * (lane & 1)] = 0ull;  // zero entries
 
   // Slices of the base level: every test holds every base value.
   for (int v = wave; v < n; v += nwaves) {
@@ -886,13 +899,22 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
   for (int round = 0;; ++round) {
     const int slot = round & 1;
     if (wave == 0) {
+      // item size: at most kSliceArcs arcs, small enough for about four
+      // items per wave when the round has few arcs
+      int arcs = 0;
+      for (int v0 = 0; v0 < n; v0 += 64) {
+        const int v = v0 + lane;
+        if (v < n && bit_of(cur, v)) arcs += net.adj_off[v + 1] - net.adj_off[v];
+      }
+      for (int d = 32; d > 0; d >>= 1) arcs += __shfl_xor(arcs, d, 64);
+      const int K = uniform_int(max(1, min(kSliceArcs, (arcs + 4 * nwaves - 1) / (4 * nwaves))));
       int base = 0, ibase = 0;
       for (int v0 = 0; v0 < n; v0 += 64) {
         const int v = v0 + lane;
         const bool f = v < n && bit_of(cur, v);
         const u64 mask = __ballot(f);
         const int rank = mbcnt(mask);
-        const int items = f ? (net.adj_off[v + 1] - net.adj_off[v] + kSliceArcs - 1) / kSliceArcs : 0;
+        const int items = f ? (net.adj_off[v + 1] - net.adj_off[v] + K - 1) / K : 0;
         const int inc = wave_inclusive_scan(items);
         if (f) {
           list[base + rank] = v;
@@ -905,6 +927,7 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
         pre[base] = ibase;
         ctl.m = base;
         ctl.total = ibase;
+        ctl.k = K;
         ctl.item_ctr = 0;
         ctl.nchanged[slot] = 0;
       }
@@ -916,6 +939,7 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
     __syncthreads();
     const int total = uniform_int(ctl.total);
     const int m = uniform_int(ctl.m);
+    const int K = uniform_int(ctl.k);
     const u64 act = (u64)uniform_ll((long long)ctl.act);
     while (true) {
       int idx = 0;
@@ -929,8 +953,8 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
         else hi = mid - 1;
       }
       const int t = uniform_int(list[lo]);
-      const int s0 = net.adj_off[t] + (idx - uniform_int(pre[lo])) * kSliceArcs;
-      const int s1 = min(s0 + kSliceArcs, net.adj_off[t + 1]);
+      const int s0 = net.adj_off[t] + (idx - uniform_int(pre[lo])) * K;
+      const int s1 = min(s0 + K, net.adj_off[t + 1]);
       const int cnt = uniform_int(s1 - s0);
       // stage the slices of t and the values alive in some live test
       const u64* slt = sl + (long long)t * dpad;
