@@ -745,7 +745,7 @@ struct SliceCtl {
 
 __host__ __device__ inline long long slice_lds_bytes(int n, int dpad, int nwaves) {
   const long long nvw = (n + 31) / 32;
-  long long b = 4ll * n + 4ll * (n + 1) + 8ll * nvw;
+  long long b = 4ll * n + 4ll * (n + 1) + 12ll * nvw;
   b = (b + 15) & ~15ll;
   b += 8ll * nwaves * (kStageWord * (dpad / 64) + CPIM_MAX_WORDS);
   return b;
@@ -790,19 +790,35 @@ __device__ __forceinline__ u64 clear_supported(u64 pend, unsigned bits, const u6
 // Revise u <- t for every live test (act).  plane_u: rows of the values of u
 // over t's domain (element (w, b) = word w of the row of b); slu: slices of
 // u; stage: slices of t in the kStageWord layout; atw[w]: values of t alive
-// in some live test.  Returns (wave-uniform) whether some bit of u was cleared
+// in some live test.  A variable no test has changed yet (touched_u false)
+// still holds `all` on its base values, so its slices are derived from the
+// base domain words base_u instead of being loaded; a stale "untouched"
+// only yields a superset of the live tests, which costs work, not accuracy.  Returns (wave-uniform) whether some bit of u was cleared
 // by this wave.
 __device__ __forceinline__ bool slice_revise(const u64* __restrict__ plane_u, u64* slu,
                                              const u64* stage, const u64* atw, int W, int dpad,
-                                             u64 act) {
+                                             u64 act, bool touched_u, const u64* __restrict__ base_u,
+                                             u64 all) {
   const int lane = lane_id();
   bool changed = false;
   for (int j0 = 0; j0 < W; j0 += 4) {
     u64 need[4], row0[4];
+    if (touched_u) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = j0 + q;
+        need[q] = j < W ? ld_dom<true>(&slu[j * 64 + lane]) : 0ull;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = j0 + q;
+        need[q] = (j < W && ((base_u[j < W ? j : 0] >> lane) & 1ull)) ? all : 0ull;
+      }
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int j = j0 + q;
-      need[q] = j < W ? ld_dom<true>(&slu[j * 64 + lane]) : 0ull;
       row0[q] = j < W ? plane_u[j * 64 + lane] : 0ull;
     }
 #pragma unroll
@@ -857,7 +873,8 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
   int* pre = list + n;
   unsigned* cur = (unsigned*)(pre + n + 1);
   unsigned* nxt = cur + nvw;
-  const long long head = (4ll * n + 4ll * (n + 1) + 8ll * nvw + 15) & ~15ll;
+  unsigned* touched = nxt + nvw;  // variables some test has changed
+  const long long head = (4ll * n + 4ll * (n + 1) + 12ll * nvw + 15) & ~15ll;
   u64* stage = (u64*)(smem + head) + (long long)wave * (kStageWord * W + CPIM_MAX_WORDS);
   u64* atw = stage + kStageWord * W;
   if (lane < 2 * W) stage[(lane >> 1) * kStageWord + 33 * (lane & 1)] = 0ull;  // zero entries
@@ -872,6 +889,7 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
   for (int i = tid; i < nvw; i += nthreads) {
     cur[i] = 0u;
     nxt[i] = 0u;
+    touched[i] = 0u;
   }
   if (tid == 0) {
     ctl.act = all;
@@ -891,6 +909,7 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
   if (tid < tc) {
     const int x = tests[tf + tid].x;
     atomicOr(&cur[x >> 5], 1u << (x & 31));
+    atomicOr(&touched[x >> 5], 1u << (x & 31));
   }
   __threadfence();
   __syncthreads();
@@ -976,7 +995,9 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
         const int pos_t = __builtin_amdgcn_readlane(e.z, i);
         const long long o = (long long)readlane64((u64)off, i);
         const u64* plane_u = net.sup + o + (long long)(1 - pos_t) * W * dpad;
-        const bool ch = slice_revise(plane_u, sl + (long long)u * dpad, stage, atw, W, dpad, act);
+        const bool tu = uniform_int((int)bit_of(touched, u));
+        const bool ch = slice_revise(plane_u, sl + (long long)u * dpad, stage, atw, W, dpad, act, tu,
+                                     base_dom + (long long)u * W, all);
         ++revs;
         if (ch && lane == 0) slice_mark(nxt, &ctl, u, slot);
       }
@@ -994,6 +1015,7 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
       alive_all &= o;
     }
     if (lane == 0 && alive_all != ~0ull) atomicAnd(&ctl.act, alive_all);
+    for (int i = tid; i < nvw; i += nthreads) touched[i] |= nxt[i];
     __syncthreads();
     const u64 act_now = (u64)uniform_ll((long long)ctl.act);
     if (uniform_int(ctl.nchanged[slot]) == 0 || act_now == 0ull) break;
