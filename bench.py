@@ -173,12 +173,11 @@ def serving(count: int, rate: float, log) -> dict:
                 continue
             srv.step()
             done_at = time.perf_counter() - t0
-            for rid in list(pending):
+            for rid in srv.poll():
                 r = srv.result(rid)
-                if r is not None:
-                    lat.append(done_at - pending.pop(rid))
-                    limited += r["status"] == 0
-                    solved += r["status"] == 1
+                lat.append(done_at - pending.pop(rid))
+                limited += r["status"] == 0
+                solved += r["status"] == 1
         wall = time.perf_counter() - t0
     finally:
         srv.close()

@@ -215,6 +215,7 @@ GPU_SIGNATURES = {
     "cpim_server_destroy": (C.c_int, [_VP]),
     "cpim_server_submit": (C.c_int, [_VP, C.POINTER(CModelDesc), C.c_int64, p_i64]),
     "cpim_server_step": (C.c_int, [_VP, p_i32]),
+    "cpim_server_poll": (C.c_int, [_VP, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32)]),
     "cpim_server_result": (C.c_int, [_VP, C.c_int64, C.POINTER(CRequestResult), p_i32]),
 }
 

@@ -1257,8 +1257,10 @@ struct UnitDesc {
   int pad;
 };
 
+// states_out[u] receives a copy of unit u's state when its launch ends, so
+// the host reads every unit's outcome with one copy.
 __global__ void k_mac_units(const UnitDesc* __restrict__ units, int count, unsigned* counter,
-                            int only, char* scratch) {
+                            int only, char* scratch, MacState* states_out) {
   extern __shared__ __align__(16) char smem[];
   __shared__ WgCtl ctl;
   __shared__ int unit_sh;
@@ -1272,6 +1274,7 @@ __global__ void k_mac_units(const UnitDesc* __restrict__ units, int count, unsig
     char* base = d.use_lds ? smem : scratch + d.scratch_off;
     WgMem mem = wg_carve(base, d.net.n, d.net.W);
     wg_mac_run(d.net, mem, &ctl, d.args);
+    if (threadIdx.x == 0) states_out[u] = *d.args.state;
     if (only >= 0) break;
   }
 }

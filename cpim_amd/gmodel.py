@@ -554,6 +554,13 @@ class Server:
         N.check(N.gpu_lib().cpim_server_step(self._h, C.byref(a)))
         return a.value
 
+    def poll(self, cap: int = 4096) -> List[int]:
+        """Ids of the requests finished since the last poll."""
+        ids = (C.c_int64 * max(1, cap))()
+        n = C.c_int32(0)
+        N.check(N.gpu_lib().cpim_server_poll(self._h, ids, cap, C.byref(n)))
+        return list(ids[:n.value])
+
     def result(self, rid: int):
         """(done, status, num_positive, num_negative, solution or None, steps)."""
         r = N.CRequestResult()

@@ -324,6 +324,9 @@ int cpim_server_submit(cpim_server* server, const cpim_model_desc* desc, int64_t
  * every resident request by nodes_per_step nodes in one launch, retire the
  * finished ones.  *active receives the number still resident or queued. */
 int cpim_server_step(cpim_server* server, int32_t* active);
+/* Ids of the requests finished since the last poll, oldest first (at most
+ * cap; the rest stay for the next poll).  No counterpart in the reference. */
+int cpim_server_poll(cpim_server* server, int64_t* ids, int32_t cap, int32_t* count);
 /* Result of a request; solution [num_vars] may be NULL.  Returns
  * CPIM_ERR_BUSY while the request is not finished. */
 int cpim_server_result(cpim_server* server, int64_t request_id, cpim_request_result* out,

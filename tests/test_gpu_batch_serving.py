@@ -83,9 +83,17 @@ def test_serving_mix_equals_alone(gpu, nodes_per_step, max_slots):
     try:
         ids = [srv.submit(r, node_limit=limit) for r in reqs]
         steps = 0
+        finished = []
         while srv.step() > 0:
+            finished += srv.poll(cap=3)
             steps += 1
             assert steps < 100000
+        while True:
+            more = srv.poll(cap=3)
+            if not more:
+                break
+            finished += more
+        assert sorted(finished) == sorted(ids)
         for rid, desc in zip(ids, reqs):
             got = srv.result(rid)
             assert got is not None
@@ -105,8 +113,13 @@ def test_serving_result_busy_until_done(gpu):
     try:
         rid = srv.submit(workload.serving_request(3), node_limit=500)
         assert srv.result(rid) is None
+        assert srv.poll() == []
+        finished = []
         while srv.step() > 0:
-            pass
+            finished += srv.poll()
+        finished += srv.poll()
+        assert finished == [rid]
         assert srv.result(rid) is not None
+        assert srv.poll() == []
     finally:
         srv.close()
