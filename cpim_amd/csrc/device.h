@@ -11,7 +11,9 @@
 //   tsup [c][pos][a][tw]        table constraints: bitset over tuple indices of the
 //                               tuples whose pos-th value is a.
 //   adj  [adj_off[v]..adj_off[v+1])  CSR of the constraints of v in constraint order:
-//                               {c, other var (binary) or -1 (table), pos of v, 0}
+//                               {c, other var (binary) or -1 (table), pos of v,
+//                                binary: plane pair index p (planes at sup + p*2*W*dpad),
+//                                table: -1}
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -216,6 +218,11 @@ __device__ __forceinline__ int revise_binary_at(const DevNet& net, u64* dom, int
   return removed;
 }
 
+// Offset in sup of the planes of a binary arc's constraint.
+__device__ __forceinline__ long long arc_plane_off(const DevNet& net, const int4& e) {
+  return (long long)e.w * 2 * net.W * net.dpad;
+}
+
 template <bool kCoherent = false>
 __device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* size, int c, int u,
                                              int t, int pos_t, bool* wiped) {
@@ -230,7 +237,29 @@ __device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* s
 // or is assigned (asg bitmap, may be NULL), are not revised.  *wiped is set if this wave
 // emptied a domain.  on_removed(var, count) is called (by the whole wave)
 // for every variable that lost values.  Requires tw <= 64 (4096 tuples).
+//
+// The OR over live values runs with lanes = (value k, word w) pairs: one
+// round of independent row loads per 64 / Tp values (Tp = tw rounded up to a
+// power of two), then an OR across the lanes of each word.  The per-value
+// check loads the value's row four words at a time.
 // ---------------------------------------------------------------------------
+// Index of the k-th set bit of the W-word domain held by lanes 0..W-1 of
+// d_l (k < popcount); lanes call it with their own k.
+__device__ __forceinline__ int kth_value(u64 d_l, int W, int k) {
+  int a = -1;
+  for (int j = 0; j < W && a < 0; ++j) {
+    u64 x = readlane64(d_l, j);
+    const int c = __popcll(x);
+    if (k < c) {
+      for (int i = 0; i < k; ++i) x &= x - 1;
+      a = j * 64 + (int)__builtin_ctzll(x);
+    } else {
+      k -= c;
+    }
+  }
+  return a;
+}
+
 template <typename OnRemoved, bool kCoherent = false>
 __device__ __forceinline__ void revise_table(const DevNet& net, u64* dom, int* size, int c,
                                              int skip_var, const unsigned* asg,
@@ -242,23 +271,28 @@ __device__ __forceinline__ void revise_table(const DevNet& net, u64* dom, int* s
   const int arity = mp->arity;
   const u64* ts = net.tsup + mp->off;
   const int tw = mp->tw;
+  const int Tp = pow2_at_least(tw);
+  const int per = 64 / Tp;  // values per round of loads
   // valid tuples, word `lane` (lanes < tw)
-  u64 valid_l = lane < tw ? ~0ull : 0ull;
+  u64 valid_l = ~0ull;
   for (int p = 0; p < arity; ++p) {
     const int v = mp->scope[p];
+    const u64 dv_l = lane < W ? ld_dom<kCoherent>(&dom[(long long)v * W + lane]) : 0ull;
+    int cnt = 0;
+    for (int j = 0; j < W; ++j) cnt += __popcll(readlane64(dv_l, j));
+    const int w = lane & (Tp - 1);
     u64 acc = 0ull;
-    for (int j = 0; j < W; ++j) {
-      u64 word = ld_dom<kCoherent>(&dom[(long long)v * W + j]);
-      word = readlane64(word, 0);
-      while (word) {
-        const int bit = (int)__builtin_ctzll(word);
-        word &= word - 1;
-        const int a = j * 64 + bit;
-        if (lane < tw) acc |= ts[((long long)p * dpad + a) * tw + lane];
+    for (int k0 = 0; k0 < cnt; k0 += per) {
+      const int k = k0 + lane / Tp;
+      if (w < tw && k < cnt) {
+        const int a = kth_value(dv_l, W, k);
+        acc |= ts[((long long)p * dpad + a) * tw + w];
       }
     }
+    for (int o = Tp; o < 64; o <<= 1) acc |= __shfl_xor(acc, o, 64);
     valid_l &= acc;
   }
+  if (lane >= tw) valid_l = 0ull;
   bool any_wipe = false;
   for (int p = 0; p < arity; ++p) {
     const int y = mp->scope[p];
@@ -273,11 +307,13 @@ __device__ __forceinline__ void revise_table(const DevNet& net, u64* dom, int* s
       bool keep = false;
       if ((duj >> lane) & 1ull) {
         const u64* row = ts + ((long long)p * dpad + b) * tw;
-        for (int w = 0; w < tw; ++w) {
-          if (row[w] & readlane64(valid_l, w)) {
-            keep = true;
-            break;
-          }
+        for (int w0 = 0; w0 < tw && !keep; w0 += 4) {
+          u64 r[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) r[q] = w0 + q < tw ? row[w0 + q] : 0ull;
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (w0 + q < tw && (r[q] & readlane64(valid_l, w0 + q < tw ? w0 + q : 0))) keep = true;
         }
       }
       const u64 mj = __ballot(keep);

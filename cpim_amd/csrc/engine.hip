@@ -256,7 +256,9 @@ ModelImage make_image(const cpim_model_desc* d, bool skip_non_binary, bool own_r
     for (int i = 0; i < m.arity; ++i) {
       const int v = m.scope[i];
       const int other = m.kind == 0 ? m.scope[1 - i] : -1;
-      im.adj[pos[v]++] = make_int4(c, other, i, 0);
+      // binary: index of the constraint's plane pair (sup offset / (2 W dpad))
+      const int pair = m.kind == 0 ? (int)(m.off / (2ll * im.W * im.dpad)) : -1;
+      im.adj[pos[v]++] = make_int4(c, other, i, pair);
     }
   }
   // level 0
@@ -580,6 +582,7 @@ int cpim_gmodel_get_info(const cpim_gmodel* m, cpim_gmodel_info* out) {
 namespace {
 
 bool fits_lds(const cpim_gmodel* m) { return wg_mem_bytes(m->dm.n, m->dm.W) <= kLdsBudget; }
+
 
 void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats* st) {
   const int L = m->current_level;
@@ -1010,8 +1013,8 @@ void run_singleton_batched(cpim_gmodel* m, const int2* tests_host, long long cou
   if (bytes <= kLdsBudget) {
     for (long long first = 0; first < count; first += INT_MAX) {
       const long long blocks = std::min<long long>(count - first, INT_MAX);
-      hipLaunchKernelGGL(k_singleton_tests<true>, dim3((unsigned)blocks), dim3(threads), (size_t)bytes, m->stream,
-                         net, m->level_dom(L), m->level_size(L), S.tests.as<int2>(), count, first,
+      hipLaunchKernelGGL(k_singleton_tests<true>, dim3((unsigned)blocks), dim3(threads), (size_t)bytes,
+                         m->stream, net, m->level_dom(L), m->level_size(L), S.tests.as<int2>(), count, first,
                          S.fails.as<unsigned char>(), S.revs.as<unsigned long long>(), (char*)nullptr, 0ll);
       HIPC(hipGetLastError());
     }
@@ -1358,7 +1361,8 @@ int cpim_gmodel_solve(cpim_gmodel* m, const cpim_solve_opts* opts, cpim_search_s
     MacState s{};
     while (true) {
       if (bytes <= kLdsBudget) {
-        hipLaunchKernelGGL(k_mac<true>, dim3(1), dim3(threads), (size_t)bytes, m->stream, net, A, (char*)nullptr);
+        hipLaunchKernelGGL(k_mac<true>, dim3(1), dim3(threads), (size_t)bytes, m->stream, net, A,
+                           (char*)nullptr);
       } else {
         char* scratch = m->get_scratch(bytes);
         hipLaunchKernelGGL(k_mac<false>, dim3(1), dim3(threads), 0, m->stream, net, A, scratch);

@@ -376,6 +376,8 @@ struct WgMem {
   unsigned* asg;  // [nvw]
   int* list;      // [n]
   int* pre;       // [n + 1]
+  const int* aoff;   // adjacency offsets (net.adj_off)
+  const int4* adj;   // adjacency entries (net.adj)
 };
 
 struct WgCtl {
@@ -384,6 +386,7 @@ struct WgCtl {
   int nchanged[2];
   int m;
   int total;
+  int k;  // arcs per work item this round
   int sel_var;
   int sel_val;
   unsigned long long revs;
@@ -408,9 +411,12 @@ __host__ __device__ inline long long wg_mem_bytes(int n, int W) {
   return (b + 15) & ~15ll;
 }
 
-__device__ __forceinline__ WgMem wg_carve(char* base, int n, int W) {
+__device__ __forceinline__ WgMem wg_carve(char* base, const DevNet& net) {
+  const int n = net.n, W = net.W;
   const int nvw = (n + 31) / 32;
   WgMem m;
+  m.aoff = net.adj_off;
+  m.adj = net.adj;
   m.dom = (u64*)base;
   char* p = base + 8ll * n * W;
   m.size = (int*)p;
@@ -443,7 +449,7 @@ __device__ __forceinline__ void wg_mark(unsigned* nxt, WgCtl* ctl, int v, int sl
 // Must be called by every thread of the workgroup.  Work items are runs of
 // up to kWgArcs arcs of one changed variable: the lanes of a wave fetch the
 // arcs' metadata together, then the wave revises them one by one.
-constexpr int kWgArcs = 8;
+constexpr int kWgArcs = 8;  // most arcs per work item
 
 __device__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_dirs_round0) {
   const int n = net.n;
@@ -460,13 +466,23 @@ __device__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_
   for (int round = 0;; ++round) {
     const int slot = round & 1;
     if (wave == 0) {
+      // item size: at most kWgArcs arcs, small enough for about two items
+      // per wave when the round has few arcs
+      int arcs = 0;
+      for (int v0 = 0; v0 < n; v0 += 64) {
+        const int v = v0 + lane;
+        if (v < n && bit_of(cur, v)) arcs += mem.aoff[v + 1] - mem.aoff[v];
+      }
+      for (int d = 32; d > 0; d >>= 1) arcs += __shfl_xor(arcs, d, 64);
+      const int nwaves = nthreads >> 6;
+      const int K = uniform_int(max(1, min(kWgArcs, (arcs + 2 * nwaves - 1) / (2 * nwaves))));
       int base = 0, ibase = 0;
       for (int v0 = 0; v0 < n; v0 += 64) {
         const int v = v0 + lane;
         const bool f = v < n && bit_of(cur, v);
         const u64 mask = __ballot(f);
         const int rank = mbcnt(mask);
-        const int items = f ? (net.adj_off[v + 1] - net.adj_off[v] + kWgArcs - 1) / kWgArcs : 0;
+        const int items = f ? (mem.aoff[v + 1] - mem.aoff[v] + K - 1) / K : 0;
         const int inc = wave_inclusive_scan(items);
         if (f) {
           mem.list[base + rank] = v;
@@ -479,6 +495,7 @@ __device__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_
         mem.pre[base] = ibase;
         ctl->m = base;
         ctl->total = ibase;
+        ctl->k = K;
         ctl->arc_ctr = 0;
         ctl->nchanged[slot] = 0;
       }
@@ -490,6 +507,7 @@ __device__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_
     __syncthreads();
     const int total = uniform_int(ctl->total);
     const int m = uniform_int(ctl->m);
+    const int K = uniform_int(ctl->k);
     const bool both = both_dirs_round0 && round == 0;
     while (true) {
       int idx = 0;
@@ -504,23 +522,17 @@ __device__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_
         else hi = mid - 1;
       }
       const int t = uniform_int(mem.list[lo]);
-      const int s0 = net.adj_off[t] + (idx - uniform_int(mem.pre[lo])) * kWgArcs;
-      const int cnt = uniform_int(min(s0 + kWgArcs, net.adj_off[t + 1]) - s0);
+      const int s0 = mem.aoff[t] + (idx - uniform_int(mem.pre[lo])) * K;
+      const int cnt = uniform_int(min(s0 + K, mem.aoff[t + 1]) - s0);
       int4 e = make_int4(0, 0, 0, 0);
-      int kind = 0;
-      long long off = 0;
-      if (lane < cnt) {
-        e = net.adj[s0 + lane];
-        kind = net.con[e.x].kind;
-        off = net.con[e.x].off;
-      }
+      if (lane < cnt) e = mem.adj[s0 + lane];
       for (int i = 0; i < cnt; ++i) {
         const int c = __builtin_amdgcn_readlane(e.x, i);
+        const int u = __builtin_amdgcn_readlane(e.y, i);
         bool wiped = false;
-        if (__builtin_amdgcn_readlane(kind, i) == 0) {
-          const int u = __builtin_amdgcn_readlane(e.y, i);
+        if (u >= 0) {  // binary
           const int pos_t = __builtin_amdgcn_readlane(e.z, i);
-          const long long o = (long long)readlane64((u64)off, i);
+          const long long o = (long long)__builtin_amdgcn_readlane(e.w, i) * 2 * net.W * net.dpad;
           if (!bit_of(mem.asg, u)) {
             const int r = revise_binary_at(net, mem.dom, mem.size, o, u, t, pos_t, &wiped);
             ++revs;
@@ -633,7 +645,7 @@ __global__ void k_block_gac(DevNet net, u64* dom, int* size, int seed_var, char*
   __shared__ WgCtl ctl;
   const int n = net.n, W = net.W;
   char* base = kLds ? smem : scratch;
-  WgMem mem = wg_carve(base, n, W);
+  WgMem mem = wg_carve(base, net);
   if (threadIdx.x == 0) ctl.revs = 0;
   wg_copy_words(mem.dom, dom, (long long)n * W);
   for (int v = threadIdx.x; v < n; v += blockDim.x) mem.size[v] = size[v];
@@ -682,7 +694,7 @@ __global__ void k_singleton_tests(DevNet net, const u64* __restrict__ base_dom,
   const int n = net.n, W = net.W;
   const int nvw = (n + 31) / 32;
   char* base = kLds ? smem : scratch + (long long)blockIdx.x * scratch_stride;
-  WgMem mem = wg_carve(base, n, W);
+  WgMem mem = wg_carve(base, net);
   const int2 tst = tests[unit];
   const int x = uniform_int(tst.x), a = uniform_int(tst.y);
   wg_copy_words(mem.dom, base_dom, (long long)n * W);
@@ -985,15 +997,11 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
       }
       __builtin_amdgcn_wave_barrier();
       int4 e = make_int4(0, 0, 0, 0);
-      long long off = 0;
-      if (lane < cnt) {
-        e = net.adj[s0 + lane];
-        off = net.con[e.x].off;
-      }
+      if (lane < cnt) e = net.adj[s0 + lane];
       for (int i = 0; i < cnt; ++i) {
         const int u = __builtin_amdgcn_readlane(e.y, i);
         const int pos_t = __builtin_amdgcn_readlane(e.z, i);
-        const long long o = (long long)readlane64((u64)off, i);
+        const long long o = (long long)__builtin_amdgcn_readlane(e.w, i) * 2 * W * dpad;
         const u64* plane_u = net.sup + o + (long long)(1 - pos_t) * W * dpad;
         const bool tu = uniform_int((int)bit_of(touched, u));
         const bool ch = slice_revise(plane_u, sl + (long long)u * dpad, stage, atw, W, dpad, act, tu,
@@ -1240,7 +1248,7 @@ __global__ void k_mac(DevNet net, MacArgs args, char* scratch) {
   extern __shared__ __align__(16) char smem[];
   __shared__ WgCtl ctl;
   char* base = kLds ? smem : scratch;
-  WgMem mem = wg_carve(base, net.n, net.W);
+  WgMem mem = wg_carve(base, net);
   wg_mac_run(net, mem, &ctl, args);
 }
 
@@ -1272,7 +1280,7 @@ __global__ void k_mac_units(const UnitDesc* __restrict__ units, int count, unsig
     if (u >= count) break;
     const UnitDesc d = units[u];
     char* base = d.use_lds ? smem : scratch + d.scratch_off;
-    WgMem mem = wg_carve(base, d.net.n, d.net.W);
+    WgMem mem = wg_carve(base, d.net);
     wg_mac_run(d.net, mem, &ctl, d.args);
     if (threadIdx.x == 0) states_out[u] = *d.args.state;
     if (only >= 0) break;
