@@ -1,0 +1,53 @@
+"""XCSP files through the device solver (needs an MI355X): the reference's
+own instances, read by the product reader, give the reference's known
+answers and the oracle's node counts, and the CLI prints the reference's
+'MAC stats:' line."""
+import os
+import re
+
+import pytest
+
+import oracle
+from cpim_amd import GModelAdapter, solve, xcsp
+
+pytestmark = pytest.mark.gpu
+
+XML = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "xcsp")
+
+
+def run_cli(capsys, *args):
+    code = solve.main(list(args))
+    return code, capsys.readouterr().out
+
+
+def test_cli_known_answers(gpu, capsys):
+    code, out = run_cli(capsys, os.path.join(XML, "test.xml"))
+    assert code == 0
+    assert "3 variables, 2 constraints" in out
+    assert "MAC solution (original values): 0 0 1" in out
+    assert re.search(r"MAC stats: time=\d+ ms, positives=3, negatives=0, nodes=3, timeout=false", out)
+    code, out = run_cli(capsys, os.path.join(XML, "queens-4_ext.xml"))
+    assert code == 0 and "MAC solution (original values): 1 3 0 2" in out
+    code, out = run_cli(capsys, os.path.join(XML, "composed-25-1-2-2_ext.xml"))
+    assert code == 1 and "did not find a solution" in out
+
+
+@pytest.mark.parametrize("fname", sorted(f for f in os.listdir(XML) if f.endswith(".xml")))
+def test_xcsp_files_match_oracle(gpu, fname):
+    inst = xcsp.read(os.path.join(XML, fname))
+    r = oracle.Oracle(inst.desc).mac(node_limit=20000)
+    with GModelAdapter.Build(inst.desc) as m:
+        st, sol, _ = m.Solve(node_limit=20000)
+    assert (st.num_solutions, st.num_positive, st.num_negative) == \
+        (r.num_sol, r.num_positive, r.num_negative)
+    if r.num_sol:
+        assert list(sol) == list(r.solution)
+
+
+def test_cli_sac_and_gac_only(gpu, capsys):
+    code, out = run_cli(capsys, os.path.join(XML, "queens-4_ext.xml"), "--gac-only")
+    assert code == 0 and "GAC: consistent" in out
+    code, out = run_cli(capsys, os.path.join(XML, "queens-4_ext.xml"), "--sac")
+    assert "SAC: consistent" in out and "MAC stats:" in out
+    code, _ = run_cli(capsys, os.path.join(XML, "missing.xml"))
+    assert code == 2

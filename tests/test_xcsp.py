@@ -1,0 +1,81 @@
+"""XCSP 2.x reader (CPU): the reference's fixture files read by the product
+reader equal the golden models, and the normalization rules of
+src/model/model_normalizer.cpp hold on hand-made instances."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from cpim_amd import xcsp
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+XML = os.path.join(HERE, "golden", "xcsp")
+FILES = sorted(f for f in os.listdir(XML) if f.endswith(".xml"))
+
+
+def as_sets(desc):
+    return [set(map(tuple, np.asarray(t).reshape(-1, len(s)).tolist()))
+            for t, s in zip(desc.tuple_lists(), desc.scopes())]
+
+
+@pytest.mark.parametrize("fname", FILES)
+def test_fixture_matches_golden_model(fname):
+    inst = xcsp.read(os.path.join(XML, fname))
+    gdesc, _ = load_golden(fname[:-4])
+    assert list(inst.desc.dom_size) == list(gdesc.dom_size)
+    assert [list(s) for s in inst.desc.scopes()] == [list(s) for s in gdesc.scopes()]
+    assert as_sets(inst.desc) == as_sets(gdesc)
+
+
+def test_test_xml_known_values():
+    inst = xcsp.read(os.path.join(XML, "test.xml"))
+    assert inst.var_names == ["V0", "V1", "V2"]
+    assert inst.var_values == [[0, 1, 2]] * 3
+    assert as_sets(inst.desc) == [{(0, 0), (1, 1), (2, 2)}, {(0, 1), (0, 2), (1, 2)}]
+
+
+DOC = """<instance><presentation name="h"/>
+<domains><domain name="D0">7 3,5 3</domain><domain name="D1">-1..1</domain></domains>
+<variables><variable name="X" domain="D0"/><variable name="Y" domain="D1"/><variable name="Z" domain="D1"/></variables>
+<relations>
+ <relation name="R0" arity="2" semantics="conflicts">3 -1|7 1|3 -1</relation>
+ <relation name="R1" arity="3" semantics="supports">5 0 1|7,1,-1</relation>
+</relations>
+<constraints>
+ <constraint name="C0" arity="2" scope="X Y" reference="R0"/>
+ <constraint name="C1" arity="3" scope="X Y Z" reference="R1"/>
+ <constraint name="C2" arity="2" scope="Y Z" reference="P0"/>
+</constraints></instance>"""
+
+
+def test_normalization_rules():
+    with pytest.raises(xcsp.XcspError, match="not an extension relation"):
+        xcsp.read(DOC)
+    inst = xcsp.read(DOC, skip_unsupported=True)
+    assert inst.skipped == ["C2"]
+    # enumerated domain sorted and deduplicated, range expanded
+    assert inst.var_values == [[3, 5, 7], [-1, 0, 1], [-1, 0, 1]]
+    c0, c1 = inst.desc.tuple_lists()
+    # conflicts -> complement over canonical values, lexicographic order
+    forbidden = {(0, 0), (2, 2)}
+    want = [(a, b) for a in range(3) for b in range(3) if (a, b) not in forbidden]
+    assert [tuple(t) for t in c0.reshape(-1, 2).tolist()] == want
+    assert [tuple(t) for t in c1.reshape(-1, 3).tolist()] == [(1, 1, 2), (2, 2, 0)]
+    assert inst.original_solution([2, 0, 1]) == [7, -1, 0]
+
+
+@pytest.mark.parametrize("bad, msg", [
+    (DOC.replace("5 0 1|", "5 0 9|"), "not present in original domain"),
+    (DOC.replace("5 0 1|", "5 0|"), "Tuple size mismatch"),
+    (DOC.replace('scope="X Y" reference="R0"', 'scope="X Q" reference="R0"'), "Variable not found"),
+    (DOC.replace(">7 3,5 3<", "><"), "Domain values are empty"),
+])
+def test_errors(bad, msg):
+    with pytest.raises(xcsp.XcspError, match=msg):
+        xcsp.read(bad, skip_unsupported=True)
+
+
+def test_expanded_conflicts_limit():
+    with pytest.raises(xcsp.XcspError, match="exceeds limit"):
+        xcsp.read(DOC, skip_unsupported=True, max_expanded_tuples=8)
