@@ -27,7 +27,9 @@ def test_cli_known_answers(gpu, capsys):
     assert "MAC solution (original values): 0 0 1" in out
     assert re.search(r"MAC stats: time=\d+ ms, positives=3, negatives=0, nodes=3, timeout=false", out)
     code, out = run_cli(capsys, os.path.join(XML, "queens-4_ext.xml"))
-    assert code == 0 and "MAC solution (original values): 1 3 0 2" in out
+    # the known answer [1, 3, 0, 2] in canonical indices; the file's domain is 1..4
+    assert code == 0 and "MAC solution (canonical indices): 1 3 0 2" in out
+    assert "MAC solution (original values): 2 4 1 3" in out
     code, out = run_cli(capsys, os.path.join(XML, "composed-25-1-2-2_ext.xml"))
     assert code == 1 and "did not find a solution" in out
 
