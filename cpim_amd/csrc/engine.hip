@@ -584,6 +584,12 @@ namespace {
 bool fits_lds(const cpim_gmodel* m) { return wg_mem_bytes(m->dm.n, m->dm.W) <= kLdsBudget; }
 
 
+unsigned long long slot_sum(const unsigned long long (&v)[kStatSlots]) {
+  unsigned long long t = 0;
+  for (unsigned long long x : v) t += x;
+  return t;
+}
+
 void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats* st) {
   const int L = m->current_level;
   u64* dom = m->level_dom(L);
@@ -624,11 +630,17 @@ void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats*
     m->sync();
     rounds = res.rounds;
   } else {
+    // about four work units per resident wave (32 per CU)
+    const long long target = 4ll * 32 * m->props.cus;
+    int parts = 1;
+    while (parts < 32 && (long long)nfw * parts < target) parts *= 2;
     const int waves_per_block = threads / 64;
-    const int grid = std::max(1, std::min((nfw + waves_per_block - 1) / waves_per_block, 8 * m->props.cus));
+    const long long units = (long long)nfw * parts;
+    const int grid = (int)std::max<long long>(1, std::min<long long>((units + waves_per_block - 1) / waves_per_block,
+                                                                      8ll * m->props.cus));
     for (int r = 0; r < max_rounds; ++r) {
       hipLaunchKernelGGL(k_gac_round, dim3(grid), dim3(threads), 0, m->stream, net, dom, size, F, nfw,
-                         ctl, r);
+                         ctl, r, parts);
       HIPC(hipGetLastError());
       HIPC(hipMemcpyAsync(&res, ctl, sizeof(res), hipMemcpyDeviceToHost, m->stream));
       m->sync();
@@ -641,10 +653,10 @@ void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats*
     }
   }
   st->iterations = rounds;
-  st->deletions = (int32_t)res.deletions;
+  st->deletions = (int32_t)slot_sum(res.deletions);
   st->inconsistent = res.wipe_round != INT_MAX;
   st->converged = st->inconsistent ? 0 : res.converged;
-  st->revisions = (int64_t)res.revisions;
+  st->revisions = (int64_t)slot_sum(res.revisions);
 }
 
 void gac_async(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
@@ -674,10 +686,10 @@ void gac_async(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
   m->sync();
   if (res.err) fail(CPIM_ERR_RUNTIME, "asynchronous fixpoint exceeded its time budget");
   st->iterations = 1;
-  st->deletions = (int32_t)res.deletions;
+  st->deletions = (int32_t)slot_sum(res.deletions);
   st->inconsistent = res.wipe != 0;
   st->converged = !res.wipe && res.pending == 0;
-  st->revisions = (int64_t)res.revisions;
+  st->revisions = (int64_t)slot_sum(res.revisions);
 }
 
 void gac_block(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
@@ -705,16 +717,20 @@ void gac_block(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
   st->revisions = (int64_t)res.revisions;
 }
 
-int auto_gac_mode(const cpim_gmodel* m) {
+int auto_gac_mode(const cpim_gmodel* m, int assigned_var) {
   // Small networks: one workgroup with the domains in LDS (one launch, no
-  // grid barrier).  Large ones: the asynchronous one-launch grid fixpoint.
+  // grid barrier).  Large ones seeded with every constraint: host-driven
+  // rounds, each spread over the whole grid (few rounds at the benchmark
+  // tightness; 0.21 ms against 0.52 ms asynchronous for one fixpoint of
+  // 80,000 revisions).  Large ones seeded from one variable: the
+  // asynchronous one-launch fixpoint, which has no per-round host sync.
   if (fits_lds(m) && m->dm.num_active <= 4096) return CPIM_GAC_BLOCK;
-  return CPIM_GAC_ASYNC;
+  return assigned_var < 0 ? CPIM_GAC_PLAIN : CPIM_GAC_ASYNC;
 }
 
 void gac(cpim_gmodel* m, int assigned_var, int mode, cpim_gac_stats* st) {
   std::memset(st, 0, sizeof(*st));
-  if (mode == CPIM_GAC_AUTO) mode = auto_gac_mode(m);
+  if (mode == CPIM_GAC_AUTO) mode = auto_gac_mode(m, assigned_var);
   if (mode == CPIM_GAC_PERSISTENT)
     fail(CPIM_ERR_UNSUPPORTED,
          "persistent GAC is switched off: its first GPU run went silent and is under investigation");

@@ -62,6 +62,10 @@ __global__ void k_build_planes(const u64* __restrict__ rel, int nb, int max_dom,
 // Termination uses epochs that only grow (last_push_round, wipe_round), so
 // every block takes the same decision after a barrier without any reset.
 // ===========================================================================
+// Statistics counters are spread over kStatSlots addresses (by block): one
+// address taking an atomic from every wave of a large grid serializes.
+constexpr int kStatSlots = 32;
+
 struct GacCtl {
   unsigned scanner[3];
   int last_push_round;  // highest round that enqueued a constraint
@@ -69,8 +73,8 @@ struct GacCtl {
   int rounds;           // rounds executed (written at the end)
   int converged;
   int pad;
-  unsigned long long deletions;
-  unsigned long long revisions;
+  unsigned long long deletions[kStatSlots];
+  unsigned long long revisions[kStatSlots];
 };
 
 __device__ __forceinline__ int ld_agent(const int* p) {
@@ -84,7 +88,8 @@ __device__ __forceinline__ void push_var(const DevNet& net, int v, unsigned* B, 
     const int c = net.adj[i].x;
     atomicOr(&B[c >> 5], 1u << (c & 31));
   }
-  if (lane == 0) atomicMax(&ctl->last_push_round, r);
+  // the epoch only grows: skip the atomic once another wave has raised it
+  if (lane == 0 && ld_agent(&ctl->last_push_round) < r) atomicMax(&ctl->last_push_round, r);
 }
 
 __device__ __forceinline__ void grid_process_constraint(const DevNet& net, u64* dom, int* size,
@@ -121,8 +126,12 @@ __device__ __forceinline__ void grid_process_constraint(const DevNet& net, u64* 
   }
 }
 
+// Work units are (frontier word, part) pairs: part p of word w covers the
+// constraint bits [p * 32 / parts, (p + 1) * 32 / parts) of A[w], so a round
+// over few words still spreads over many waves; waves take units in a fixed
+// stride.
 __device__ __forceinline__ void grid_round(const DevNet& net, u64* dom, int* size, unsigned* F,
-                                           int nfw, GacCtl* ctl, int r) {
+                                           int nfw, GacCtl* ctl, int r, int parts) {
   unsigned* A = F + (long long)(r % 3) * nfw;
   unsigned* B = F + (long long)((r + 1) % 3) * nfw;
   unsigned* Cl = F + (long long)((r + 2) % 3) * nfw;
@@ -132,13 +141,18 @@ __device__ __forceinline__ void grid_round(const DevNet& net, u64* dom, int* siz
     Cl[i] = 0u;
   const int lane = lane_id();
   unsigned long long revs = 0, dels = 0;
-  while (true) {
-    int wi = 0;
-    if (lane == 0) wi = (int)atomicAdd(&ctl->scanner[r % 3], 1u);
-    wi = __shfl(wi, 0, 64);
-    if (wi >= nfw) break;
+  const long long units = (long long)nfw * parts;
+  const int pbits = 32 / parts;
+  // static stride over the units: one counter for tens of thousands of units
+  // would serialize on its atomic
+  const long long nwaves_grid = (long long)gridDim.x * (blockDim.x >> 6);
+  for (long long ui = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); ui < units;
+       ui += nwaves_grid) {
     if (ld_agent(&ctl->wipe_round) <= r) break;
-    unsigned bits = A[wi];
+    const int wi = (int)(ui / parts);
+    const int part = ui - wi * parts;
+    const unsigned pmask = pbits == 32 ? ~0u : (((1u << pbits) - 1u) << (part * pbits));
+    unsigned bits = A[wi] & pmask;
     while (bits) {
       const int b = __ffs(bits) - 1;
       bits &= bits - 1u;
@@ -146,15 +160,15 @@ __device__ __forceinline__ void grid_round(const DevNet& net, u64* dom, int* siz
     }
   }
   if (lane == 0 && (revs | dels)) {
-    atomicAdd(&ctl->revisions, revs);
-    atomicAdd(&ctl->deletions, dels);
+    atomicAdd(&ctl->revisions[blockIdx.x % kStatSlots], revs);
+    atomicAdd(&ctl->deletions[blockIdx.x % kStatSlots], dels);
   }
 }
 
 // One round per launch (host-driven fixpoint).
 __global__ void k_gac_round(DevNet net, u64* dom, int* size, unsigned* F, int nfw, GacCtl* ctl,
-                            int r) {
-  grid_round(net, dom, size, F, nfw, ctl, r);
+                            int r, int parts) {
+  grid_round(net, dom, size, F, nfw, ctl, r, parts);
 }
 
 // Every round in one cooperative launch.
@@ -164,7 +178,7 @@ __global__ void k_gac_persistent(DevNet net, u64* dom, int* size, unsigned* F, i
   int r = 0;
   int converged = 0;
   for (; r < max_rounds; ++r) {
-    grid_round(net, dom, size, F, nfw, ctl, r);
+    grid_round(net, dom, size, F, nfw, ctl, r, 1);
     grid.sync();
     const int wr = ld_agent(&ctl->wipe_round);
     const int lp = ld_agent(&ctl->last_push_round);
@@ -216,8 +230,8 @@ struct AsyncCtl {
   unsigned long long pending;
   int wipe;
   int err;
-  unsigned long long deletions;
-  unsigned long long revisions;
+  unsigned long long deletions[kStatSlots];
+  unsigned long long revisions[kStatSlots];
 };
 
 __device__ __forceinline__ void push_var_async(const DevNet& net, int v, unsigned* F, AsyncCtl* ctl) {
@@ -317,8 +331,8 @@ __global__ void k_gac_async(DevNet net, u64* dom, int* size, unsigned* F, int nf
     }
   }
   if (lane == 0 && (revs | dels)) {
-    atomicAdd(&ctl->revisions, revs);
-    atomicAdd(&ctl->deletions, dels);
+    atomicAdd(&ctl->revisions[blockIdx.x % kStatSlots], revs);
+    atomicAdd(&ctl->deletions[blockIdx.x % kStatSlots], dels);
   }
 }
 
