@@ -215,6 +215,8 @@ GPU_SIGNATURES = {
     "cpim_server_destroy": (C.c_int, [_VP]),
     "cpim_server_submit": (C.c_int, [_VP, C.POINTER(CModelDesc), C.c_int64, p_i64]),
     "cpim_server_step": (C.c_int, [_VP, p_i32]),
+    "cpim_tune_set": (C.c_int, [C.c_char_p, C.c_int64]),
+    "cpim_tune_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64)]),
     "cpim_server_poll": (C.c_int, [_VP, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32)]),
     "cpim_server_result": (C.c_int, [_VP, C.c_int64, C.POINTER(CRequestResult), p_i32]),
 }
@@ -287,3 +289,14 @@ def check(code: int) -> None:
         if code == CPIM_ERR_INVALID_ARGUMENT:
             raise CpimInvalidArgument(code, text)
         raise CpimError(code, text)
+
+
+def tune_set(name: str, value: int) -> None:
+    """Set an engine tuning knob (0 = automatic); see include/cpim_gpu.h."""
+    check(gpu_lib().cpim_tune_set(name.encode(), int(value)))
+
+
+def tune_get(name: str) -> int:
+    v = C.c_int64(0)
+    check(gpu_lib().cpim_tune_get(name.encode(), C.byref(v)))
+    return v.value

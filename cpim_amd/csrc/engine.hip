@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cstdio>
@@ -1060,18 +1061,56 @@ void run_singleton_batched(cpim_gmodel* m, const int2* tests_host, long long cou
 // 64 (a variable with more values is split into balanced groups; small
 // variables share a group), one workgroup per group.  The slices of the
 // resident groups take n * dpad words each, within kSliceBudgetBytes.
+// ---------------------------------------------------------------------------
+// Tuning knobs (cpim_tune_set / cpim_tune_get), process-wide; 0 = automatic.
+// Their first values come from the environment.
+// ---------------------------------------------------------------------------
+struct Knob {
+  const char* name;
+  const char* env;
+  std::atomic<int> value;
+  bool loaded;
+};
+Knob g_knobs[] = {
+    {"sac_slice_masks", "CPIM_SLICE_MASKS", {0}, false},
+    {"sac_slice_threads", "CPIM_SLICE_THREADS", {0}, false},
+    {"sac_slice_kernel", "CPIM_SLICE_KERNEL", {0}, false},
+    {"unit_threads", "CPIM_UNIT_THREADS", {0}, false},
+};
+std::mutex g_knob_mu;
+
+bool knob_valid(const char* name, long long v) {
+  const std::string k(name);
+  if (k == "sac_slice_masks") return v == 0 || v == 1 || v == 2 || v == 4;
+  if (k == "sac_slice_kernel") return v == 0 || v == 1;
+  return v == 0 || (v >= 64 && v <= 1024 && v % 64 == 0);
+}
+
+Knob* find_knob(const char* name) {
+  for (Knob& k : g_knobs)
+    if (std::strcmp(k.name, name) == 0) {
+      std::lock_guard<std::mutex> lock(g_knob_mu);
+      if (!k.loaded) {
+        const char* e = std::getenv(k.env);
+        const long long v = e ? std::atoll(e) : 0;
+        k.value = knob_valid(name, v) ? (int)v : 0;
+        k.loaded = true;
+      }
+      return &k;
+    }
+  return nullptr;
+}
+
+int knob(const char* name) { return find_knob(name)->value.load(); }
+
+// Bit-sliced singleton tests.  Two kernels: k_sliced_wide<M> (default: 64 M
+// tests per workgroup, one shared stage per changed variable) and
+// k_sliced_tests (64 tests, one stage per wave; sac_slice_kernel = 1).
 constexpr int kSliceThreads = 512;
 
-// Workgroup size of the bit-sliced kernel: 512 threads (best of 128..1024 on
-// the 500-variable, 256-value SAC pass), or CPIM_SLICE_THREADS
-// (64..1024, a multiple of 64) for tuning.
 int slice_threads() {
-  static const int t = [] {
-    const char* e = std::getenv("CPIM_SLICE_THREADS");
-    const int v = e ? std::atoi(e) : kSliceThreads;
-    return (v >= 64 && v <= 1024 && v % 64 == 0) ? v : kSliceThreads;
-  }();
-  return t;
+  const int t = knob("sac_slice_threads");
+  return t ? t : kSliceThreads;
 }
 constexpr long long kSliceBudgetBytes = 4ll << 30;
 
@@ -1083,8 +1122,24 @@ int slice_threads_for(const cpim_gmodel* m) {
   return slice_lds_bytes(m->dm.n, m->dm.dpad, t / 64) <= kLdsBudget ? t : 0;
 }
 
+// Masks per (variable, value) of the wide kernel: sac_slice_masks, or 4 when
+// the tests fill a workgroup of 256 per CU, 2 when 128 do, else 1; halved
+// while the LDS image does not fit (0 if M = 1 does not).
+int wide_masks_for(const cpim_gmodel* m, long long count) {
+  int M = knob("sac_slice_masks");
+  if (M == 0) {
+    const long long cus = std::max(1, m->props.cus);
+    M = count >= 256 * cus ? 4 : (count >= 128 * cus ? 2 : 1);
+  }
+  while (M > 1 && wide_lds_bytes(m->dm.n, m->dm.W, M) > kLdsBudget) M /= 2;
+  return wide_lds_bytes(m->dm.n, m->dm.W, M) <= kLdsBudget ? M : 0;
+}
+
+bool use_wide_kernel() { return knob("sac_slice_kernel") == 0; }
+
 bool sliced_eligible(const cpim_gmodel* m) {
-  return (int)m->im.bin_cons.size() == m->im.num_active && slice_threads_for(m) > 0;
+  if ((int)m->im.bin_cons.size() != m->im.num_active) return false;
+  return use_wide_kernel() ? wide_masks_for(m, 1) > 0 : slice_threads_for(m) > 0;
 }
 
 void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long count, unsigned char* fail_host,
@@ -1106,6 +1161,10 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
     for (long long i = 0; i < count; ++i) sorted_buf[(size_t)i] = tests_host[order[(size_t)i]];
     sorted = sorted_buf.data();
   }
+  const bool wide = use_wide_kernel();
+  const int M = wide ? wide_masks_for(m, count) : 1;
+  REQUIRE(M > 0, "bit-sliced singleton tests do not fit in LDS");
+  const long long cap = 64ll * M;  // tests per workgroup
   std::vector<int> gfirst;
   gfirst.reserve((size_t)(count / 32 + 2));
   int open = -1;  // start of the group being filled, -1 if none
@@ -1113,11 +1172,11 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
     long long j = i;
     while (j < count && sorted[j].x == sorted[i].x) ++j;
     const long long len = j - i;
-    if (len > 64) {
+    if (len > cap) {
       open = -1;
-      const long long k = (len + 63) / 64;
+      const long long k = (len + cap - 1) / cap;
       for (long long p = 0; p < k; ++p) gfirst.push_back((int)(i + len * p / k));
-    } else if (open < 0 || i + len - open > 64) {
+    } else if (open < 0 || i + len - open > cap) {
       gfirst.push_back((int)i);
       open = (int)i;
     }
@@ -1146,20 +1205,37 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
     HIPC(hipMemcpyAsync(S.perm.p, order.data(), sizeof(int) * count, hipMemcpyHostToDevice, m->stream));
   HIPC(hipMemcpyAsync(S.gfirst.p, gfirst.data(), sizeof(int) * (groups + 1), hipMemcpyHostToDevice, m->stream));
   HIPC(hipMemsetAsync(S.revs.p, 0, sizeof(unsigned long long), m->stream));
-  const long long slice_bytes = 8ll * m->dm.n * m->dm.dpad;
+  const long long slice_bytes = 8ll * m->dm.n * m->dm.dpad * M;
   const long long per_launch = std::max<long long>(1, std::min<long long>(groups, kSliceBudgetBytes / slice_bytes));
   u64* slices = reinterpret_cast<u64*>(m->get_scratch(slice_bytes * per_launch));
   const int L = m->current_level;
   DevNet net = m->dm.net();
-  const int threads = slice_threads_for(m);
-  const size_t lds = (size_t)slice_lds_bytes(m->dm.n, m->dm.dpad, threads / 64);
+  int threads = 0;
+  size_t lds = 0;
+  if (wide) {
+    // 16 waves from 128 tests up, 8 below; at least one thread per test, at
+    // most the kernel's launch bound
+    threads = knob("sac_slice_threads") ? knob("sac_slice_threads") : (M >= 2 ? 1024 : 512);
+    threads = std::min(std::max(threads, 64 * M), kWideMaxThreads);
+    lds = (size_t)wide_lds_bytes(m->dm.n, m->dm.W, M);
+  } else {
+    threads = slice_threads_for(m);
+    lds = (size_t)slice_lds_bytes(m->dm.n, m->dm.dpad, threads / 64);
+  }
   const int* perm = sorted_in ? nullptr : S.perm.as<int>();
   HIPC(hipEventRecord(S.ev[0], m->stream));
   for (long long g0 = 0; g0 < groups; g0 += per_launch) {
     const long long blocks = std::min(per_launch, groups - g0);
-    hipLaunchKernelGGL(k_sliced_tests, dim3((unsigned)blocks), dim3(threads), lds, m->stream, net,
-                       m->level_dom(L), S.stests.as<int2>(), S.gfirst.as<int>(), g0, perm,
-                       S.fails.as<unsigned char>(), S.revs.as<unsigned long long>(), slices);
+    const dim3 grid((unsigned)blocks), block(threads);
+    const u64* base = m->level_dom(L);
+    const int2* tl = S.stests.as<int2>();
+    const int* gf = S.gfirst.as<int>();
+    unsigned char* fl = S.fails.as<unsigned char>();
+    unsigned long long* rv = S.revs.as<unsigned long long>();
+    if (!wide) hipLaunchKernelGGL(k_sliced_tests, grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices);
+    else if (M == 1) hipLaunchKernelGGL(k_sliced_wide<1>, grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices);
+    else if (M == 2) hipLaunchKernelGGL(k_sliced_wide<2>, grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices);
+    else hipLaunchKernelGGL(k_sliced_wide<4>, grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices);
     HIPC(hipGetLastError());
   }
   HIPC(hipEventRecord(S.ev[1], m->stream));
@@ -1248,6 +1324,25 @@ std::vector<u64> read_level(cpim_gmodel* m, int level) {
 }  // namespace
 
 extern "C" {
+
+int cpim_tune_set(const char* name, int64_t value) {
+  return guarded([&] {
+    REQUIRE(name, "null argument");
+    Knob* k = find_knob(name);
+    REQUIRE(k, "unknown tuning knob");
+    REQUIRE(knob_valid(name, value), "value out of range for this knob");
+    k->value = (int)value;
+  });
+}
+
+int cpim_tune_get(const char* name, int64_t* value) {
+  return guarded([&] {
+    REQUIRE(name && value, "null argument");
+    Knob* k = find_knob(name);
+    REQUIRE(k, "unknown tuning knob");
+    *value = k->value.load();
+  });
+}
 
 int cpim_gmodel_singleton_tests(cpim_gmodel* m, const int32_t* tests, int64_t count, int32_t mode, uint8_t* fail_out,
                                 cpim_sac_stats* st) {

@@ -1052,6 +1052,296 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
 }
 
 // ---------------------------------------------------------------------------
+// Wide bit-sliced singleton tests: 64 M tests per workgroup (M = 1, 2, 4),
+// sl[(v * dpad + b) * M + m] holding tests 64m..64m+63.  Within a round the
+// workgroup walks the changed variables t one at a time: all threads stage
+// t's masks into a shared, double-buffered LDS area (one barrier per t),
+// then the waves split t's arcs u <- t.  One support entry then serves 64 M
+// tests; the revision is the one of k_sliced_tests applied to M masks at
+// once (same fixpoint argument, same exact end-of-round wipe-out check).
+// ---------------------------------------------------------------------------
+template <int M>
+struct WideCtl {
+  int m;
+  int nchanged[2];
+  int pad;
+  u64 act[M];
+  unsigned long long revs;
+};
+
+// LDS bytes: list [n], cur/nxt/touched bitmaps, two stage buffers of
+// kStageWord * W entries of M masks, two alive-value rows of W words.
+__host__ __device__ inline long long wide_lds_bytes(int n, int W, int M) {
+  const long long nvw = (n + 31) / 32;
+  long long b = 4ll * n + 12ll * nvw;
+  b = (b + 15) & ~15ll;
+  b += 2ll * 8 * M * kStageWord * W + 2ll * 8 * W;
+  return b;
+}
+
+template <int M>
+__device__ __forceinline__ bool any_mask(const u64 (&x)[M]) {
+  u64 o = 0ull;
+#pragma unroll
+  for (int m = 0; m < M; ++m) o |= x[m];
+  return o != 0ull;
+}
+
+// pend[m] &= ~(st[a][m] for every set bit a of bits), four bits per step;
+// entry -1 of st is zero (exhausted bits).  Entries hold the M masks of one
+// value together (two 16-byte reads for M = 4).
+template <int M>
+__device__ __forceinline__ void clear_supported_wide(u64 (&pend)[M], unsigned bits, const u64* st) {
+  while (bits && any_mask<M>(pend)) {
+    const int a0 = ffbl_raw(bits);
+    bits &= bits - 1u;
+    const int a1 = ffbl_raw(bits);
+    bits &= bits - 1u;
+    const int a2 = ffbl_raw(bits);
+    bits &= bits - 1u;
+    const int a3 = ffbl_raw(bits);
+    bits &= bits - 1u;
+    const u64* e0 = st + (long long)a0 * M;
+    const u64* e1 = st + (long long)a1 * M;
+    const u64* e2 = st + (long long)a2 * M;
+    const u64* e3 = st + (long long)a3 * M;
+#pragma unroll
+    for (int m = 0; m < M; ++m) pend[m] &= ~((e0[m] | e1[m]) | (e2[m] | e3[m]));
+  }
+}
+
+template <int M>
+__device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64* slu, const u64* stage,
+                                            const u64* atw, int W, int dpad, const u64 (&act)[M],
+                                            bool touched_u, const u64* __restrict__ base_u,
+                                            const u64 (&all)[M]) {
+  const int lane = lane_id();
+  bool changed = false;
+  for (int j = 0; j < W; ++j) {
+    const int b = j * 64 + lane;
+    u64 pend[M];
+    if (touched_u) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) pend[m] = ld_dom<true>(&slu[(long long)b * M + m]) & act[m];
+    } else {
+      const bool in = (base_u[j] >> lane) & 1ull;
+#pragma unroll
+      for (int m = 0; m < M; ++m) pend[m] = in ? (all[m] & act[m]) : 0ull;
+    }
+    const u64 row0 = plane_u[b];
+    if (any_mask<M>(pend)) {
+      const u64 r = row0 & atw[0];
+      clear_supported_wide<M>(pend, (unsigned)r, stage + (long long)1 * M);
+      clear_supported_wide<M>(pend, (unsigned)(r >> 32), stage + (long long)34 * M);
+    }
+    for (int w = 1; w < W && any_mask<M>(pend); ++w) {
+      const u64 r = plane_u[(long long)w * dpad + b] & atw[w];
+      const u64* st = stage + (long long)w * kStageWord * M;
+      clear_supported_wide<M>(pend, (unsigned)r, st + (long long)1 * M);
+      clear_supported_wide<M>(pend, (unsigned)(r >> 32), st + (long long)34 * M);
+    }
+    if (any_mask<M>(pend)) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        if (pend[m]) {
+          const u64 old = atomicAnd(&slu[(long long)b * M + m], ~pend[m]);
+          if (old & pend[m]) changed = true;
+        }
+      }
+    }
+  }
+  return __ballot(changed) != 0ull;
+}
+
+// Most threads per workgroup of k_sliced_wide, and the waves per SIMD its
+// register budget is set for: 8 (64 VGPRs) keeps two 1024-thread workgroups
+// on every CU.  With 4 masks this spills a few registers and is still the
+// fastest choice measured (11.5 ms against 13.6 ms at 512 threads and 80
+// VGPRs for the 128,000-test pass).
+constexpr int kWideMaxThreads = 1024;
+constexpr int kWideWavesPerSimd = 8;
+
+template <int M>
+__global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_wide(DevNet net, const u64* __restrict__ base_dom,
+                              const int2* __restrict__ tests, const int* __restrict__ gfirst,
+                              long long g0, const int* __restrict__ perm, unsigned char* fail,
+                              unsigned long long* revs_out, u64* slices) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ WideCtl<M> ctl;
+  const int n = net.n, W = net.W, dpad = net.dpad;
+  const int nvw = (n + 31) / 32;
+  const int tid = threadIdx.x, nthreads = blockDim.x;
+  const int lane = lane_id();
+  const int wave = uniform_int(tid >> 6), nwaves = nthreads >> 6;
+  const int B = gridDim.x, bid = blockIdx.x;
+  const int xcd = bid & 7, q = B >> 3, rem = B & 7;
+  const long long g = g0 + (long long)xcd * q + (xcd < rem ? xcd : rem) + (bid >> 3);
+  const int tf = gfirst[g];
+  const int tc = uniform_int(gfirst[g + 1] - tf);
+  u64 all[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int k = tc - 64 * m;
+    all[m] = k >= 64 ? ~0ull : (k > 0 ? ((1ull << k) - 1ull) : 0ull);
+  }
+  u64* sl = slices + (long long)bid * n * dpad * M;
+
+  int* list = (int*)smem;
+  unsigned* cur = (unsigned*)(list + n);
+  unsigned* nxt = cur + nvw;
+  unsigned* touched = nxt + nvw;
+  const long long head = (4ll * n + 12ll * nvw + 15) & ~15ll;
+  u64* stage0 = (u64*)(smem + head);
+  const long long stage_words = (long long)M * kStageWord * W;
+  const int nbuf = 2;  // double buffer
+  u64* atw0 = stage0 + nbuf * stage_words;
+
+  // slices of the base level: every test holds every base value
+  for (int v = wave; v < n; v += nwaves)
+    for (int b = lane; b < dpad; b += 64) {
+      const bool in = (base_dom[(long long)v * W + (b >> 6)] >> (b & 63)) & 1ull;
+#pragma unroll
+      for (int m = 0; m < M; ++m) sl[((long long)v * dpad + b) * M + m] = in ? all[m] : 0ull;
+    }
+  for (int i = tid; i < nvw; i += nthreads) {
+    cur[i] = 0u;
+    nxt[i] = 0u;
+    touched[i] = 0u;
+  }
+  // zero entries of every stage buffer
+  for (int i = tid; i < nbuf * 2 * W * M; i += nthreads) {
+    const int m = i % M, k = i / M;  // k = buffer * 2W + entry
+    const int buf = k / (2 * W), e = k % (2 * W);
+    stage0[buf * stage_words + ((long long)(e >> 1) * kStageWord + 33 * (e & 1)) * M + m] = 0ull;
+  }
+  if (tid == 0) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) ctl.act[m] = all[m];
+    ctl.revs = 0;
+  }
+  __threadfence();
+  __syncthreads();
+  // test k reduces its variable x_k to {a_k}
+  for (int k = wave; k < tc; k += nwaves) {
+    const int2 tk = tests[tf + k];
+    const int m = k >> 6;
+    const u64 bit = 1ull << (k & 63);
+    for (int b = lane; b < dpad; b += 64) {
+      u64* p = &sl[((long long)tk.x * dpad + b) * M + m];
+      if (b == tk.y) atomicOr(p, bit);
+      else atomicAnd(p, ~bit);
+    }
+  }
+  if (tid < tc) {
+    const int x = tests[tf + tid].x;
+    atomicOr(&cur[x >> 5], 1u << (x & 31));
+    atomicOr(&touched[x >> 5], 1u << (x & 31));
+  }
+  __threadfence();
+  __syncthreads();
+
+  unsigned long long revs = 0;
+  for (int round = 0;; ++round) {
+    const int slot = round & 1;
+    if (wave == 0) {
+      int base = 0;
+      for (int v0 = 0; v0 < n; v0 += 64) {
+        const int v = v0 + lane;
+        const bool f = v < n && bit_of(cur, v);
+        const u64 mask = __ballot(f);
+        if (f) list[base + mbcnt(mask)] = v;
+        base += __popcll(mask);
+      }
+      if (lane == 0) {
+        ctl.m = base;
+        ctl.nchanged[slot] = 0;
+      }
+    } else {
+      for (int i = tid - 64; i < nvw; i += nthreads - 64) nxt[i] = 0u;
+    }
+    if (nthreads == 64)
+      for (int i = lane; i < nvw; i += 64) nxt[i] = 0u;
+    __syncthreads();
+    const int mcount = uniform_int(ctl.m);
+    u64 act[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) act[m] = (u64)uniform_ll((long long)ctl.act[m]);
+    for (int i = 0; i < mcount; ++i) {
+      const int t = uniform_int(list[i]);
+      u64* stage = stage0 + (i & 1) * stage_words;
+      u64* atw = atw0 + (i & 1) * W;
+      // stage t: thread v holds value v's M masks; one wave covers one word
+      const u64* slt = sl + (long long)t * dpad * M;
+      for (int v = tid; v < dpad; v += nthreads) {
+        u64 alive = 0ull;
+        u64* dst = stage + (long long)stage_pos(v) * M;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          const u64 x = ld_dom<true>(&slt[(long long)v * M + m]);
+          dst[m] = x;
+          alive |= x & act[m];
+        }
+        const u64 word = __ballot(alive != 0ull);
+        if (lane == 0) atw[v >> 6] = word;
+      }
+      __syncthreads();
+      const int a0 = net.adj_off[t], deg = net.adj_off[t + 1] - a0;
+      for (int j = wave; j < deg; j += nwaves) {
+        const int4 e = net.adj[a0 + j];
+        const int u = uniform_int(e.y);
+        const int pos_t = uniform_int(e.z);
+        const long long o = (long long)uniform_int(e.w) * 2 * W * dpad;
+        const u64* plane_u = net.sup + o + (long long)(1 - pos_t) * W * dpad;
+        const bool tu = uniform_int((int)bit_of(touched, u));
+        const bool ch = wide_revise<M>(plane_u, sl + (long long)u * dpad * M, stage, atw, W, dpad, act, tu,
+                                       base_dom + (long long)u * W, all);
+        ++revs;
+        if (ch && lane == 0) {
+          const unsigned bit = 1u << (u & 31);
+          const unsigned old = atomicOr(&nxt[u >> 5], bit);
+          if (!(old & bit)) atomicAdd(&ctl.nchanged[slot], 1);
+        }
+      }
+    }
+    __threadfence();
+    __syncthreads();
+    // exact wipe-out check: a test wiped out at v only if v changed
+    u64 alive_all[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) alive_all[m] = ~0ull;
+    for (int v = wave; v < n; v += nwaves) {
+      if (!uniform_int((int)bit_of(nxt, v))) continue;
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        u64 o = 0ull;
+        for (int b = lane; b < dpad; b += 64) o |= ld_dom<true>(&sl[((long long)v * dpad + b) * M + m]);
+        for (int d = 32; d > 0; d >>= 1) o |= __shfl_xor(o, d, 64);
+        alive_all[m] &= o;
+      }
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int m = 0; m < M; ++m)
+        if (alive_all[m] != ~0ull) atomicAnd(&ctl.act[m], alive_all[m]);
+    }
+    for (int i = tid; i < nvw; i += nthreads) touched[i] |= nxt[i];
+    __syncthreads();
+    u64 act_any = 0ull;
+#pragma unroll
+    for (int m = 0; m < M; ++m) act_any |= (u64)uniform_ll((long long)ctl.act[m]);
+    if (uniform_int(ctl.nchanged[slot]) == 0 || act_any == 0ull) break;
+    unsigned* tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+  if (lane == 0 && revs) atomicAdd(&ctl.revs, revs);
+  __syncthreads();
+  if (tid < tc)
+    fail[perm ? perm[tf + tid] : tf + tid] = ((ctl.act[tid >> 6] >> (tid & 63)) & 1ull) ? 0 : 1;
+  if (tid == 0 && ctl.revs) atomicAdd(revs_out, ctl.revs);
+}
+
+// ---------------------------------------------------------------------------
 // Device MAC (binary branching, dom-min over undecided variables with the
 // lowest id on ties, min value), CPU MAC::enforce semantics
 // (src/MAC.cpp:100-165).  The trail holds the domain of every depth: a

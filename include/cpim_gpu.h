@@ -201,6 +201,18 @@ typedef struct cpim_sac_stats {
   double kernel_ms;
 } cpim_sac_stats;
 
+/* Tuning knobs of the engine, process-wide; 0 = automatic.  No counterpart in
+ * the reference.  Names and values:
+ *   "sac_slice_masks"   masks per (variable, value) of the bit-sliced SAC kernel: 0, 1, 2, 4
+ *                       (64 tests per mask)
+ *   "sac_slice_threads" threads per workgroup of that kernel: 0 or 64..1024 (multiple of 64)
+ *   "sac_slice_kernel"  0 = one shared stage per changed variable, 1 = one stage per wave
+ *   "unit_threads"      threads per workgroup of the batch / serving MAC kernel
+ * First values come from CPIM_SLICE_MASKS, CPIM_SLICE_THREADS, CPIM_SLICE_KERNEL,
+ * CPIM_UNIT_THREADS. */
+int cpim_tune_set(const char* name, int64_t value);
+int cpim_tune_get(const char* name, int64_t* value);
+
 #define CPIM_SAC_BATCHED 0   /* every singleton test of a pass on the device: SLICED when the network
                                 allows it (binary constraints only), else WORKGROUP */
 #define CPIM_SAC_PLAIN 1     /* one host-driven fixpoint per singleton test */

@@ -149,3 +149,21 @@ def test_serving_request_shape():
     assert (m.con_arity == 3).sum() == 4
     lists = m.tuple_lists()
     assert all(len(lists[c]) == 500 for c in range(m.num_cons) if m.con_arity[c] == 3)
+
+
+def test_tuning_knobs():
+    from cpim_amd import _native as N
+    for name in ("sac_slice_masks", "sac_slice_threads", "sac_slice_kernel", "unit_threads"):
+        old = N.tune_get(name)
+        N.tune_set(name, 0)
+        assert N.tune_get(name) == 0
+        N.tune_set(name, old)
+    N.tune_set("sac_slice_masks", 4)
+    assert N.tune_get("sac_slice_masks") == 4
+    N.tune_set("sac_slice_masks", 0)
+    with pytest.raises(N.CpimInvalidArgument, match="out of range"):
+        N.tune_set("sac_slice_masks", 3)
+    with pytest.raises(N.CpimInvalidArgument, match="out of range"):
+        N.tune_set("unit_threads", 100)
+    with pytest.raises(N.CpimInvalidArgument, match="unknown tuning knob"):
+        N.tune_set("no_such_knob", 1)

@@ -245,8 +245,31 @@ SLICED = [  # (n, d, e, p, preremove, tests)
 ]
 
 
+SLICE_VARIANTS = [(0, 0), (0, 1), (0, 2), (0, 4), (1, 0)]  # (kernel knob, masks knob)
+
+
+@pytest.fixture(params=SLICE_VARIANTS, ids=lambda v: f"kernel{v[0]}-masks{v[1]}")
+def slice_variant(request):
+    kern, masks = request.param
+    N.tune_set("sac_slice_kernel", kern)
+    N.tune_set("sac_slice_masks", masks)
+    yield request.param
+    N.tune_set("sac_slice_kernel", 0)
+    N.tune_set("sac_slice_masks", 0)
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_golden_sac_slice_variants(gpu, name, slice_variant):
+    desc, g = load_golden(name)
+    with GModelAdapter.Build(desc) as m:
+        st = m.EnforceSAC(mode="sliced")
+        assert st.consistent == bool(g["sac_ok"])
+        if g["sac_ok"]:
+            assert (m.ReadDomains(0) == g["sac_dom"]).all()
+
+
 @pytest.mark.parametrize("case", range(len(SLICED)))
-def test_sliced_arbitrary_test_lists(gpu, case):
+def test_sliced_arbitrary_test_lists(gpu, case, slice_variant):
     n, d, e, p, pre, k = SLICED[case]
     desc = workload.random_binary(n, d, e, p, 1700 + case, preremove=pre)
     o = oracle.Oracle(desc)
@@ -369,6 +392,15 @@ def test_batched_sac_full_size_sample(gpu):
         for mode in SAC_MODES:
             got, sst = m.SingletonTests(sample, mode=mode)
             assert (got == want).all(), mode
+        for kern, masks in SLICE_VARIANTS:
+            N.tune_set("sac_slice_kernel", kern)
+            N.tune_set("sac_slice_masks", masks)
+            try:
+                got, sst = m.SingletonTests(sample, mode="sliced")
+            finally:
+                N.tune_set("sac_slice_kernel", 0)
+                N.tune_set("sac_slice_masks", 0)
+            assert (got == want).all(), (kern, masks)
         # every singleton test of the full pass, in one launch
         allf, sst = m.SingletonTests(tests)
         assert sst.singleton_tests == len(tests)
