@@ -58,7 +58,7 @@ def parse():
 
 
 # committed rocprofv3 summaries of the pass kernel, per SAC mode
-PROFILES = {"sliced": os.path.join(REPO, "profiles", "k_sliced_tests_batched_sac.txt"),
+PROFILES = {"sliced": os.path.join(REPO, "profiles", "k_sliced_wide_batched_sac.txt"),
             "workgroup": os.path.join(REPO, "profiles", "k_singleton_tests_batched_sac.txt")}
 
 
@@ -294,7 +294,8 @@ def main():
             "workload": "batched SAC1 singleton tests, random binary CSP n=500 d=256 e=20000 "
                         "tightness 0.5, one SAC pass per step",
             "singleton_tests_per_step": ntests,
-            "sac_kernel": "k_sliced_tests" if args.sac_mode == "sliced" else "k_singleton_tests",
+            "sac_kernel": "k_sliced_wide<4> (256 tests per workgroup)" if args.sac_mode == "sliced"
+                          else "k_singleton_tests",
             "parallelism": "single GPU",
         },
         "roofline": {
