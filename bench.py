@@ -58,8 +58,7 @@ def parse():
 
 
 # committed rocprofv3 summaries of the pass kernel, per SAC mode
-PROFILES = {"sliced": os.path.join(REPO, "profiles", "k_sliced_wide_batched_sac.txt"),
-            "workgroup": os.path.join(REPO, "profiles", "k_singleton_tests_batched_sac.txt")}
+PROFILES = {"sliced": os.path.join(REPO, "profiles", "k_sliced_wide_batched_sac.txt")}
 
 
 def profiled_traffic(mode: str) -> float | None:
