@@ -1076,6 +1076,7 @@ Knob g_knobs[] = {
     {"sac_slice_threads", "CPIM_SLICE_THREADS", {0}, false},
     {"sac_slice_kernel", "CPIM_SLICE_KERNEL", {0}, false},
     {"unit_threads", "CPIM_UNIT_THREADS", {0}, false},
+    {"sac_slice_nibbles", "CPIM_SLICE_NIBBLES", {0}, false},
 };
 std::mutex g_knob_mu;
 
@@ -1083,6 +1084,7 @@ bool knob_valid(const char* name, long long v) {
   const std::string k(name);
   if (k == "sac_slice_masks") return v == 0 || v == 1 || v == 2 || v == 4;
   if (k == "sac_slice_kernel") return v == 0 || v == 1;
+  if (k == "sac_slice_nibbles") return v == 0 || v == 1 || v == 2;
   return v == 0 || (v >= 64 && v <= 1024 && v % 64 == 0);
 }
 
@@ -1137,6 +1139,15 @@ int wide_masks_for(const cpim_gmodel* m, long long count) {
 
 bool use_wide_kernel() { return knob("sac_slice_kernel") == 0; }
 
+// Nibble tables (sac_slice_nibbles: 0 auto = on when their LDS image stays
+// within 80 KB, two workgroups per CU; 1 on when it fits; 2 off).
+bool use_nibble_tables(const cpim_gmodel* m, int M) {
+  const int k = knob("sac_slice_nibbles");
+  if (k == 2) return false;
+  const long long b = wide_lds_bytes(m->dm.n, m->dm.W, M, true);
+  return b <= (k == 1 ? (long long)m->props.max_lds : std::min<long long>(m->props.max_lds, 80 * 1024));
+}
+
 bool sliced_eligible(const cpim_gmodel* m) {
   if ((int)m->im.bin_cons.size() != m->im.num_active) return false;
   return use_wide_kernel() ? wide_masks_for(m, 1) > 0 : slice_threads_for(m) > 0;
@@ -1163,6 +1174,7 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
   }
   const bool wide = use_wide_kernel();
   const int M = wide ? wide_masks_for(m, count) : 1;
+  const bool nib = wide && use_nibble_tables(m, M);
   REQUIRE(M > 0, "bit-sliced singleton tests do not fit in LDS");
   const long long cap = 64ll * M;  // tests per workgroup
   std::vector<int> gfirst;
@@ -1217,7 +1229,7 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
     // most the kernel's launch bound
     threads = knob("sac_slice_threads") ? knob("sac_slice_threads") : (M >= 2 ? 1024 : 512);
     threads = std::min(std::max(threads, 64 * M), kWideMaxThreads);
-    lds = (size_t)wide_lds_bytes(m->dm.n, m->dm.W, M);
+    lds = (size_t)wide_lds_bytes(m->dm.n, m->dm.W, M, nib);
   } else {
     threads = slice_threads_for(m);
     lds = (size_t)slice_lds_bytes(m->dm.n, m->dm.dpad, threads / 64);
@@ -1233,9 +1245,12 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
     unsigned char* fl = S.fails.as<unsigned char>();
     unsigned long long* rv = S.revs.as<unsigned long long>();
     if (!wide) hipLaunchKernelGGL(k_sliced_tests, grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices);
-    else if (M == 1) hipLaunchKernelGGL(k_sliced_wide<1>, grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices);
-    else if (M == 2) hipLaunchKernelGGL(k_sliced_wide<2>, grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices);
-    else hipLaunchKernelGGL(k_sliced_wide<4>, grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices);
+#define CPIM_WIDE(MM, NB) \
+  hipLaunchKernelGGL((k_sliced_wide<MM, NB>), grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices)
+    else if (M == 1) { if (nib) CPIM_WIDE(1, true); else CPIM_WIDE(1, false); }
+    else if (M == 2) { if (nib) CPIM_WIDE(2, true); else CPIM_WIDE(2, false); }
+    else { if (nib) CPIM_WIDE(4, true); else CPIM_WIDE(4, false); }
+#undef CPIM_WIDE
     HIPC(hipGetLastError());
   }
   HIPC(hipEventRecord(S.ev[1], m->stream));

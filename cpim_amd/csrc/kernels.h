@@ -1069,13 +1069,19 @@ struct WideCtl {
   unsigned long long revs;
 };
 
-// LDS bytes: list [n], cur/nxt/touched bitmaps, two stage buffers of
-// kStageWord * W entries of M masks, two alive-value rows of W words.
-__host__ __device__ inline long long wide_lds_bytes(int n, int W, int M) {
+// Words of one stage buffer: kStageWord * W entries of M masks, or with
+// nibble tables (nib) 16 W nibbles x 16 subsets of M masks.
+__host__ __device__ inline long long wide_stage_words(int W, int M, bool nib) {
+  return nib ? 256ll * W * M : (long long)kStageWord * W * M;
+}
+
+// LDS bytes: list [n], cur/nxt/touched bitmaps, two stage buffers, two
+// alive-value rows of W words.
+__host__ __device__ inline long long wide_lds_bytes(int n, int W, int M, bool nib = false) {
   const long long nvw = (n + 31) / 32;
   long long b = 4ll * n + 12ll * nvw;
   b = (b + 15) & ~15ll;
-  b += 2ll * 8 * M * kStageWord * W + 2ll * 8 * W;
+  b += 2ll * 8 * wide_stage_words(W, M, nib) + 2ll * 8 * W;
   return b;
 }
 
@@ -1110,7 +1116,28 @@ __device__ __forceinline__ void clear_supported_wide(u64 (&pend)[M], unsigned bi
   }
 }
 
+// Nibble tables ("four Russians"): entry (q, s) of a half-word table holds,
+// for the 4 values 4q..4q+3 of the half, the OR of the masks of the values
+// in subset s.  One lookup covers every support of a row nibble.  An
+// exhausted half reads entry (7, 0) (v_ffbl_b32 gives -1, & 28 gives nibble
+// 7, and the subset of an empty word is 0), which is zero like every s = 0.
 template <int M>
+__device__ __forceinline__ void clear_supported_nib(u64 (&pend)[M], unsigned bits, const u64* tab) {
+  while (bits && any_mask<M>(pend)) {
+    const unsigned p0 = (unsigned)ffbl_raw(bits) & 28u;
+    const unsigned s0 = (bits >> p0) & 15u;
+    bits &= ~(15u << p0);
+    const unsigned p1 = (unsigned)ffbl_raw(bits) & 28u;
+    const unsigned s1 = (bits >> p1) & 15u;
+    bits &= ~(15u << p1);
+    const u64* e0 = tab + (long long)((p0 << 2) | s0) * M;
+    const u64* e1 = tab + (long long)((p1 << 2) | s1) * M;
+#pragma unroll
+    for (int m = 0; m < M; ++m) pend[m] &= ~(e0[m] | e1[m]);
+  }
+}
+
+template <int M, bool kNib>
 __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64* slu, const u64* stage,
                                             const u64* atw, int W, int dpad, const u64 (&act)[M],
                                             bool touched_u, const u64* __restrict__ base_u,
@@ -1131,14 +1158,25 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
     const u64 row0 = plane_u[b];
     if (any_mask<M>(pend)) {
       const u64 r = row0 & atw[0];
-      clear_supported_wide<M>(pend, (unsigned)r, stage + (long long)1 * M);
-      clear_supported_wide<M>(pend, (unsigned)(r >> 32), stage + (long long)34 * M);
+      if (kNib) {
+        clear_supported_nib<M>(pend, (unsigned)r, stage);
+        clear_supported_nib<M>(pend, (unsigned)(r >> 32), stage + 128ll * M);
+      } else {
+        clear_supported_wide<M>(pend, (unsigned)r, stage + (long long)1 * M);
+        clear_supported_wide<M>(pend, (unsigned)(r >> 32), stage + (long long)34 * M);
+      }
     }
     for (int w = 1; w < W && any_mask<M>(pend); ++w) {
       const u64 r = plane_u[(long long)w * dpad + b] & atw[w];
-      const u64* st = stage + (long long)w * kStageWord * M;
-      clear_supported_wide<M>(pend, (unsigned)r, st + (long long)1 * M);
-      clear_supported_wide<M>(pend, (unsigned)(r >> 32), st + (long long)34 * M);
+      if (kNib) {
+        const u64* tb = stage + 256ll * w * M;
+        clear_supported_nib<M>(pend, (unsigned)r, tb);
+        clear_supported_nib<M>(pend, (unsigned)(r >> 32), tb + 128ll * M);
+      } else {
+        const u64* st = stage + (long long)w * kStageWord * M;
+        clear_supported_wide<M>(pend, (unsigned)r, st + (long long)1 * M);
+        clear_supported_wide<M>(pend, (unsigned)(r >> 32), st + (long long)34 * M);
+      }
     }
     if (any_mask<M>(pend)) {
 #pragma unroll
@@ -1161,7 +1199,7 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
 constexpr int kWideMaxThreads = 1024;
 constexpr int kWideWavesPerSimd = 8;
 
-template <int M>
+template <int M, bool kNib>
 __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_wide(DevNet net, const u64* __restrict__ base_dom,
                               const int2* __restrict__ tests, const int* __restrict__ gfirst,
                               long long g0, const int* __restrict__ perm, unsigned char* fail,
@@ -1192,7 +1230,7 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
   unsigned* touched = nxt + nvw;
   const long long head = (4ll * n + 12ll * nvw + 15) & ~15ll;
   u64* stage0 = (u64*)(smem + head);
-  const long long stage_words = (long long)M * kStageWord * W;
+  const long long stage_words = wide_stage_words(W, M, kNib);
   const int nbuf = 2;  // double buffer
   u64* atw0 = stage0 + nbuf * stage_words;
 
@@ -1208,8 +1246,8 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
     nxt[i] = 0u;
     touched[i] = 0u;
   }
-  // zero entries of every stage buffer
-  for (int i = tid; i < nbuf * 2 * W * M; i += nthreads) {
+  // zero entries of every stage buffer (single-value layout only)
+  for (int i = tid; !kNib && i < nbuf * 2 * W * M; i += nthreads) {
     const int m = i % M, k = i / M;  // k = buffer * 2W + entry
     const int buf = k / (2 * W), e = k % (2 * W);
     stage0[buf * stage_words + ((long long)(e >> 1) * kStageWord + 33 * (e & 1)) * M + m] = 0ull;
@@ -1272,13 +1310,31 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
       u64* atw = atw0 + (i & 1) * W;
       // stage t: thread v holds value v's M masks; one wave covers one word
       const u64* slt = sl + (long long)t * dpad * M;
+      if (kNib) {
+        // nibble tables: item (q, m) ORs the 16 subsets of values 4q..4q+3
+        for (int it = tid; it < 16 * W * M; it += nthreads) {
+          const int q = it / M, mm = it - q * M;
+          u64 x[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) x[i] = ld_dom<true>(&slt[(long long)(4 * q + i) * M + mm]);
+          u64* e = stage + (long long)q * 16 * M + mm;
+#pragma unroll
+          for (int sub = 0; sub < 16; ++sub) {
+            u64 o = 0ull;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              if (sub & (1 << i)) o |= x[i];
+            e[(long long)sub * M] = o;
+          }
+        }
+      }
       for (int v = tid; v < dpad; v += nthreads) {
         u64 alive = 0ull;
         u64* dst = stage + (long long)stage_pos(v) * M;
 #pragma unroll
         for (int m = 0; m < M; ++m) {
           const u64 x = ld_dom<true>(&slt[(long long)v * M + m]);
-          dst[m] = x;
+          if (!kNib) dst[m] = x;
           alive |= x & act[m];
         }
         const u64 word = __ballot(alive != 0ull);
@@ -1293,8 +1349,8 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
         const long long o = (long long)uniform_int(e.w) * 2 * W * dpad;
         const u64* plane_u = net.sup + o + (long long)(1 - pos_t) * W * dpad;
         const bool tu = uniform_int((int)bit_of(touched, u));
-        const bool ch = wide_revise<M>(plane_u, sl + (long long)u * dpad * M, stage, atw, W, dpad, act, tu,
-                                       base_dom + (long long)u * W, all);
+        const bool ch = wide_revise<M, kNib>(plane_u, sl + (long long)u * dpad * M, stage, atw, W, dpad, act,
+                                             tu, base_dom + (long long)u * W, all);
         ++revs;
         if (ch && lane == 0) {
           const unsigned bit = 1u << (u & 31);

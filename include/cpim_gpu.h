@@ -207,9 +207,11 @@ typedef struct cpim_sac_stats {
  *                       (64 tests per mask)
  *   "sac_slice_threads" threads per workgroup of that kernel: 0 or 64..1024 (multiple of 64)
  *   "sac_slice_kernel"  0 = one shared stage per changed variable, 1 = one stage per wave
+ *   "sac_slice_nibbles" nibble (four-Russians) tables in the shared stage: 0 = when they fit
+ *                       in 80 KB of LDS, 1 = whenever they fit, 2 = never
  *   "unit_threads"      threads per workgroup of the batch / serving MAC kernel
  * First values come from CPIM_SLICE_MASKS, CPIM_SLICE_THREADS, CPIM_SLICE_KERNEL,
- * CPIM_UNIT_THREADS. */
+ * CPIM_SLICE_NIBBLES, CPIM_UNIT_THREADS. */
 int cpim_tune_set(const char* name, int64_t value);
 int cpim_tune_get(const char* name, int64_t* value);
 

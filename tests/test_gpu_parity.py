@@ -245,17 +245,22 @@ SLICED = [  # (n, d, e, p, preremove, tests)
 ]
 
 
-SLICE_VARIANTS = [(0, 0), (0, 1), (0, 2), (0, 4), (1, 0)]  # (kernel knob, masks knob)
+# (kernel, masks, nibbles) knobs: default, each mask width with and without
+# nibble tables, and the one-stage-per-wave kernel
+SLICE_VARIANTS = [(0, 0, 0), (0, 1, 1), (0, 2, 1), (0, 4, 1), (0, 1, 2), (0, 2, 2), (0, 4, 2), (1, 0, 0)]
+SLICE_KNOBS = ("sac_slice_kernel", "sac_slice_masks", "sac_slice_nibbles")
 
 
-@pytest.fixture(params=SLICE_VARIANTS, ids=lambda v: f"kernel{v[0]}-masks{v[1]}")
+def set_slice_knobs(values):
+    for k, v in zip(SLICE_KNOBS, values):
+        N.tune_set(k, v)
+
+
+@pytest.fixture(params=SLICE_VARIANTS, ids=lambda v: f"kernel{v[0]}-masks{v[1]}-nib{v[2]}")
 def slice_variant(request):
-    kern, masks = request.param
-    N.tune_set("sac_slice_kernel", kern)
-    N.tune_set("sac_slice_masks", masks)
+    set_slice_knobs(request.param)
     yield request.param
-    N.tune_set("sac_slice_kernel", 0)
-    N.tune_set("sac_slice_masks", 0)
+    set_slice_knobs((0, 0, 0))
 
 
 @pytest.mark.parametrize("name", golden_names())
@@ -392,15 +397,13 @@ def test_batched_sac_full_size_sample(gpu):
         for mode in SAC_MODES:
             got, sst = m.SingletonTests(sample, mode=mode)
             assert (got == want).all(), mode
-        for kern, masks in SLICE_VARIANTS:
-            N.tune_set("sac_slice_kernel", kern)
-            N.tune_set("sac_slice_masks", masks)
+        for variant in SLICE_VARIANTS:
+            set_slice_knobs(variant)
             try:
                 got, sst = m.SingletonTests(sample, mode="sliced")
             finally:
-                N.tune_set("sac_slice_kernel", 0)
-                N.tune_set("sac_slice_masks", 0)
-            assert (got == want).all(), (kern, masks)
+                set_slice_knobs((0, 0, 0))
+            assert (got == want).all(), variant
         # every singleton test of the full pass, in one launch
         allf, sst = m.SingletonTests(tests)
         assert sst.singleton_tests == len(tests)
