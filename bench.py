@@ -40,6 +40,8 @@ def parse():
                     help="singleton tests timed on the plain one-per-launch path")
     ap.add_argument("--cpu-sample", type=int, default=600,
                     help="singleton tests timed on the CPU oracle (about 10-15 s)")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="threads of the all-cores CPU baseline (the GPU host's CPU share is 16)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--sac-mode", default="sliced", choices=["sliced", "workgroup"],
@@ -261,6 +263,7 @@ def main():
         log(f"plain one-per-launch: {plain:,.0f} fixpoints/s on {k} tests")
 
     cpu = None
+    cpu_all = None
     if not args.skip_cpu:
         import oracle
         o = oracle.Oracle(desc)
@@ -275,6 +278,17 @@ def main():
                "sample": f"{k} singleton tests of the same pass (AC3bit restatement, "
                          f"{cdt:.1f} s single-threaded)"}
         log(f"cpu oracle: {cpu['value']:.1f} fixpoints/s")
+        # the same restatement on every thread of the host share
+        th = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        km = min(ntests, k * th)
+        idx = np.linspace(0, ntests - 1, km).astype(np.int64)
+        tc = time.perf_counter()
+        cf, _ = o.singleton_tests(tests[idx], nthreads=th)
+        cdt = time.perf_counter() - tc
+        assert (cf == fail[idx]).all(), "device and CPU oracle disagree"
+        cpu_all = {"value": km / cdt, "unit": "fixpoints/s", "cores": th, "kind": "port",
+                   "sample": f"{km} singleton tests of the same pass, {th} threads, {cdt:.1f} s"}
+        log(f"cpu oracle, {th} threads: {cpu_all['value']:.1f} fixpoints/s")
 
     out = {
         "metric": METRIC,
@@ -293,8 +307,8 @@ def main():
             "workload": "batched SAC1 singleton tests, random binary CSP n=500 d=256 e=20000 "
                         "tightness 0.5, one SAC pass per step",
             "singleton_tests_per_step": ntests,
-            "sac_kernel": "k_sliced_wide<4> (256 tests per workgroup)" if args.sac_mode == "sliced"
-                          else "k_singleton_tests",
+            "sac_kernel": "k_sliced_wide<4, nibble tables> (256 tests per workgroup)"
+                          if args.sac_mode == "sliced" else "k_singleton_tests",
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -306,6 +320,7 @@ def main():
             "traffic": None,
         },
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
         "plain_fixpoints_per_s": plain,
         "speedup_vs_plain": (value / plain) if plain else None,
         "kernel_ms_per_step": avg_kernel_ms,
