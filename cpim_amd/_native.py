@@ -34,6 +34,8 @@ CPIM_SAC_WORKGROUP = 2
 CPIM_SAC_SLICED = 3
 
 CPIM_BRANCH_BINARY = 0
+CPIM_VARH_DOM_MIN = 0
+CPIM_VARH_DOM_DEG = 1
 
 p_i32 = C.POINTER(C.c_int32)
 p_i64 = C.POINTER(C.c_int64)
@@ -117,6 +119,8 @@ class CSolveOpts(C.Structure):
         ("node_limit", C.c_int64),
         ("branching", C.c_int32),
         ("trace_capacity", C.c_int32),
+        ("var_heuristic", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -6,7 +6,8 @@
 // reference's "MAC stats:" line.
 //
 //   cpim_solve --n 30 --d 16 --e 120 --p 0.4 --seed 1 [--node_limit K]
-//              [--time_limit MS] [--sac] [--dway] [--validate-only]
+//              [--time_limit MS] [--varh dom|domdeg] [--sac] [--dway]
+//              [--validate-only]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,6 +28,7 @@ int main(int argc, char** argv) {
   long long node_limit = 0;
   int time_limit = 0;
   bool sac = false, dway = false, validate_only = false;
+  int varh = CPIM_VARH_DOM_MIN;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     auto next = [&]() { return i + 1 < argc ? argv[++i] : (char*)"0"; };
@@ -37,7 +39,15 @@ int main(int argc, char** argv) {
     else if (a == "--seed") seed = std::strtoull(next(), nullptr, 10);
     else if (a == "--node_limit") node_limit = std::atoll(next());
     else if (a == "--time_limit") time_limit = std::atoi(next());
-    else if (a == "--sac") sac = true;
+    else if (a == "--varh") {
+      const std::string h = next();
+      if (h == "dom") varh = CPIM_VARH_DOM_MIN;
+      else if (h == "domdeg") varh = CPIM_VARH_DOM_DEG;
+      else {
+        std::fprintf(stderr, "--varh takes dom or domdeg\n");
+        return 2;
+      }
+    } else if (a == "--sac") sac = true;
     else if (a == "--dway") dway = true;
     else if (a == "--validate-only") validate_only = true;
     else {
@@ -86,11 +96,20 @@ int main(int argc, char** argv) {
   }
   cpim::GModelSolver solver(&model, false,
                             dway ? cpim::GModelSolver::Branching::kDway : cpim::GModelSolver::Branching::kBinary);
+  if (varh != CPIM_VARH_DOM_MIN) {
+    try {
+      solver.SetVarHeuristic(varh);
+    } catch (const std::invalid_argument& ex) {
+      std::fprintf(stderr, "%s\n", ex.what());
+      return 2;
+    }
+  }
   cpim::GpuSearchStatistics st;
   if (node_limit > 0 && !dway) {
     cpim_solve_opts o{};
     o.node_limit = node_limit;
     o.time_limit_ms = time_limit;
+    o.var_heuristic = varh;
     cpim_search_stats cs{};
     std::vector<int32_t> sol(n, -1);
     int32_t tl = 0;

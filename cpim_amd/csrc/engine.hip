@@ -129,6 +129,7 @@ struct ModelImage {
   std::vector<u64> tsup;
   long long sup_words = 0;
   std::vector<u64> init_dom;          // [n][W]
+  std::vector<int> nbr;               // [n] distinct neighbours over the active constraints
 };
 
 ModelImage make_image(const cpim_model_desc* d, bool skip_non_binary, bool own_rel = false) {
@@ -262,6 +263,22 @@ ModelImage make_image(const cpim_model_desc* d, bool skip_non_binary, bool own_r
       im.adj[pos[v]++] = make_int4(c, other, i, pair);
     }
   }
+  // neighbour counts (Network::get_neighbor, src/Network.cpp:467-474)
+  im.nbr.assign(im.n, 0);
+  {
+    std::vector<int> seen(im.n, -1);
+    for (int v = 0; v < im.n; ++v)
+      for (int k = im.adj_off[v]; k < im.adj_off[v + 1]; ++k) {
+        const auto& m = im.con[im.adj[k].x];
+        for (int i = 0; i < m.arity; ++i) {
+          const int y = m.scope[i];
+          if (y != v && seen[y] != v) {
+            seen[y] = v;
+            ++im.nbr[v];
+          }
+        }
+      }
+  }
   // level 0
   im.init_dom.assign(nw, 0ull);
   for (int v = 0; v < im.n; ++v)
@@ -303,7 +320,7 @@ struct DevAlloc {
 
 struct DevModel {
   int n = 0, W = 0, dpad = 0, max_dom = 0, num_active = 0;
-  DevAlloc adj_off, adj, con, sup, tsup, cap;
+  DevAlloc adj_off, adj, con, sup, tsup, cap, nbr;
   long long bytes = 0;
   DevNet net() const {
     DevNet d;
@@ -339,6 +356,7 @@ void build_device_model(const ModelImage& im, DevModel& dm, hipStream_t s) {
   upload(dm.con, im.con, s, dm.bytes);
   upload(dm.tsup, im.tsup, s, dm.bytes);
   upload(dm.cap, im.cap, s, dm.bytes);
+  upload(dm.nbr, im.nbr, s, dm.bytes);
   dm.sup = DevAlloc(sizeof(u64) * im.sup_words);
   dm.bytes += sizeof(u64) * im.sup_words;
   const int nb = (int)im.bin_cons.size();
@@ -1450,6 +1468,9 @@ int cpim_gmodel_solve(cpim_gmodel* m, const cpim_solve_opts* opts, cpim_search_s
     cpim_solve_opts o{};
     if (opts) o = *opts;
     REQUIRE(o.branching == CPIM_BRANCH_BINARY, "the device solver implements binary branching only");
+    REQUIRE(o.var_heuristic == CPIM_VARH_DOM_MIN || o.var_heuristic == CPIM_VARH_DOM_DEG,
+            "unknown variable heuristic");
+    REQUIRE(o.reserved == 0, "cpim_solve_opts.reserved must be 0");
     REQUIRE(o.trace_capacity >= 0, "negative trace capacity");
     REQUIRE(o.trace_capacity == 0 || trace, "trace buffer is null");
     REQUIRE(m->current_level == 0, "the device solver starts from level 0");
@@ -1473,6 +1494,7 @@ int cpim_gmodel_solve(cpim_gmodel* m, const cpim_solve_opts* opts, cpim_search_s
     A.step_budget = 0;
     A.deadline = 0;
     A.root_out = nullptr;
+    A.nbr = o.var_heuristic == CPIM_VARH_DOM_DEG ? m->dm.nbr.as<int>() : nullptr;
     DevNet net = m->dm.net();
     const long long bytes = wg_mem_bytes(n, m->dm.W);
     const int threads = 1024;

@@ -1435,6 +1435,7 @@ struct MacArgs {
   long long step_budget;  // per launch (0 = none)
   u64 deadline;           // wall_clock64 ticks (0 = none)
   u64* root_out;          // [n][W] root-fixpoint domains, or NULL
+  const int* nbr;         // [n] neighbour counts: dom/deg ordering; NULL = dom-min
 };
 
 enum { MAC_INIT = 0, MAC_RUNNING = 1, MAC_SAT = 2, MAC_UNSAT = 3, MAC_LIMIT = 4 };
@@ -1512,11 +1513,18 @@ __device__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* ctl, const MacAr
       __syncthreads();
       break;
     }
-    // select: min |D| over undecided variables, lowest id on ties
+    // select: min key over undecided variables, lowest id on ties.  dom-min:
+    // key = |D|; dom/deg: |D| / neighbours (integer), -1 without neighbours,
+    // biased by one so that the packed key stays unsigned.
     u64 key = ~0ull;
     for (int v = threadIdx.x; v < n; v += blockDim.x)
       if (!bit_of(mem.asg, v)) {
-        const u64 k = ((u64)(unsigned)mem.size[v] << 32) | (unsigned)v;
+        unsigned hi = (unsigned)mem.size[v];
+        if (A.nbr) {
+          const int g = A.nbr[v];
+          hi = g == 0 ? 0u : (unsigned)(mem.size[v] / g) + 1u;
+        }
+        const u64 k = ((u64)hi << 32) | (unsigned)v;
         key = k < key ? k : key;
       }
     key = wg_reduce_min_u64(ctl, key);

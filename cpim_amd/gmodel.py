@@ -28,6 +28,15 @@ _SAC_MODES = {"auto": N.CPIM_SAC_BATCHED, "plain": N.CPIM_SAC_PLAIN,
               "workgroup": N.CPIM_SAC_WORKGROUP, "sliced": N.CPIM_SAC_SLICED}
 
 
+_VAR_HEURISTICS = {"dom": N.CPIM_VARH_DOM_MIN, "dom/deg": N.CPIM_VARH_DOM_DEG}
+
+
+def _var_heuristic(name: str) -> int:
+    if name not in _VAR_HEURISTICS:
+        raise ValueError("heuristic must be one of %s" % sorted(_VAR_HEURISTICS))
+    return _VAR_HEURISTICS[name]
+
+
 def _sac_mode(batched: bool, mode: Optional[str]) -> int:
     if mode is not None:
         if mode not in _SAC_MODES:
@@ -261,13 +270,18 @@ class GModel:
                               st.revisions, st.kernel_ms)
 
     # -- device search -------------------------------------------------------------
-    def Solve(self, time_limit_ms: int = 0, node_limit: int = 0, trace_capacity: int = 0):
-        """Device MAC (binary branching, CPU MAC semantics) from level 0."""
+    def Solve(self, time_limit_ms: int = 0, node_limit: int = 0, trace_capacity: int = 0,
+              heuristic: str = "dom"):
+        """Device MAC (binary branching, CPU MAC semantics) from level 0.
+
+        heuristic: "dom" (smallest domain, MAC.cpp:356-364) or "dom/deg"
+        (domain size over neighbour count, MAC.cpp:371-384)."""
         o = N.CSolveOpts()
         o.time_limit_ms = time_limit_ms
         o.node_limit = node_limit
         o.branching = N.CPIM_BRANCH_BINARY
         o.trace_capacity = trace_capacity
+        o.var_heuristic = _var_heuristic(heuristic)
         st = N.CSearchStats()
         sol = np.full(self.num_vars, -1, np.int32)
         tr = (N.CNodeRecord * max(1, trace_capacity))()
@@ -299,14 +313,19 @@ class GModelSolver:
     reference GModelSolver's d-way search (src/GModelSolver.cu:104-189) through
     the host-visible level API."""
 
-    def __init__(self, model: GModel, verbose: bool = False, branching: str = "binary"):
+    def __init__(self, model: GModel, verbose: bool = False, branching: str = "binary",
+                 heuristic: str = "dom"):
         if model is None:
             raise RuntimeError("[GModelSolver] model is null!")
         if branching not in ("binary", "dway"):
             raise ValueError("branching must be 'binary' or 'dway'")
+        _var_heuristic(heuristic)
+        if branching == "dway" and heuristic != "dom":
+            raise ValueError("the d-way search orders variables by smallest domain only")
         self.model_ = model
         self.verbose_ = verbose
         self.branching = branching
+        self.heuristic = heuristic
         self.find_all_solutions_ = False
         self.max_solutions_ = 1
         self.solutions_: List[List[int]] = []
@@ -330,7 +349,8 @@ class GModelSolver:
               trace_capacity: int = 0) -> GpuSearchStatistics:
         self.solutions_ = []
         if self.branching == "binary":
-            st, sol, self.trace = self.model_.Solve(time_limit, node_limit, trace_capacity)
+            st, sol, self.trace = self.model_.Solve(time_limit, node_limit, trace_capacity,
+                                                    self.heuristic)
             if sol is not None:
                 self.solutions_.append([int(x) for x in sol])
             return st

@@ -222,12 +222,23 @@ class GModelSolver {
     if (!model_) throw std::runtime_error("[GModelSolver] model is null!");
   }
 
+  // CPIM_VARH_DOM_MIN (default) or CPIM_VARH_DOM_DEG; the d-way search
+  // orders by smallest domain only.
+  void SetVarHeuristic(int h) {
+    if (h != CPIM_VARH_DOM_MIN && h != CPIM_VARH_DOM_DEG)
+      throw std::invalid_argument("unknown variable heuristic");
+    if (h != CPIM_VARH_DOM_MIN && branching_ == Branching::kDway)
+      throw std::invalid_argument("the d-way search orders variables by smallest domain only");
+    var_heuristic_ = h;
+  }
+
   GpuSearchStatistics Solve(int time_limit = 0) {
     solutions_.clear();
     if (branching_ == Branching::kDway) return SolveDway(time_limit);
     cpim_solve_opts o{};
     o.time_limit_ms = time_limit;
     o.branching = CPIM_BRANCH_BINARY;
+    o.var_heuristic = var_heuristic_;
     cpim_search_stats st{};
     std::vector<int32_t> sol(model_->num_vars(), -1);
     int32_t tl = 0;
@@ -320,6 +331,7 @@ class GModelSolver {
   GModel* model_;
   bool verbose_;
   Branching branching_;
+  int var_heuristic_ = CPIM_VARH_DOM_MIN;
   bool find_all_ = false;
   int max_solutions_ = 1;
   std::vector<std::vector<int>> solutions_;

@@ -79,6 +79,7 @@ def lib():
         L.orc_bf_mac.argtypes = [C.POINTER(N.CModelDesc), C.c_int64, C.POINTER(OrcStats), N.p_i32,
                                  C.POINTER(N.CNodeRecord), C.c_int32, N.p_i32]
         L.orc_set_fault.argtypes = [C.c_int]
+        L.orc_set_var_heuristic.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -190,6 +191,21 @@ def bf_mac(desc, node_limit: int = 0, trace_cap: int = 0) -> MacResult:
              for r in tr[:min(tl.value, trace_cap)]]
     return MacResult(int(st.num_sol), int(st.num_positive), int(st.num_negative),
                      bool(st.time_out), sol if st.num_sol else None, trace, st)
+
+
+class var_heuristic:
+    """Context manager selecting the oracle's variable ordering
+    (0 = dom-min, 1 = dom/deg) for orc_mac and orc_bf_mac."""
+
+    def __init__(self, varh: int):
+        self.varh = varh
+
+    def __enter__(self):
+        lib().orc_set_var_heuristic(self.varh)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_var_heuristic(0)
 
 
 class fault:

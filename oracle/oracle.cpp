@@ -13,6 +13,46 @@
 namespace {
 
 int g_fault = 0;
+int g_varh = 0;  // 0 = dom-min (VRH_DOM_MIN), 1 = dom/deg (VRH_DOM_DEG_MIN)
+
+// Number of distinct other variables sharing a constraint with each variable
+// (Network::get_neighbor, src/Network.cpp:467-474).
+std::vector<int> neighbour_counts(int n, const std::vector<std::vector<int>>& scope) {
+  std::vector<std::vector<int>> nb(n);
+  for (const auto& sc : scope)
+    for (int x : sc)
+      for (int y : sc)
+        if (x != y) nb[x].push_back(y);
+  std::vector<int> cnt(n, 0);
+  for (int v = 0; v < n; ++v) {
+    std::sort(nb[v].begin(), nb[v].end());
+    cnt[v] = (int)(std::unique(nb[v].begin(), nb[v].end()) - nb[v].begin());
+  }
+  return cnt;
+}
+
+// MAC::select_var (src/MAC.cpp:353-385): the first variable (lowest id) with
+// the smallest key among the undecided ones.  dom-min: key = |D(v)|.
+// dom/deg: key = |D(v)| / |neighbours(v)| in integer division, or -1 for a
+// variable without neighbours.  size(v) gives |D(v)|; skip(v) the decided.
+// planted: whether planted fault 2 (last of the tied variables) applies; the
+// brute-force driver passes false so that it stays an independent check.
+template <typename Size, typename Skip>
+int select_var_h(int n, const std::vector<int>& nbr, Size size, Skip skip, bool planted) {
+  int var = -1;
+  long long best = 0;
+  for (int v = 0; v < n; ++v) {
+    if (skip(v)) continue;
+    const int sz = size(v);
+    long long key = sz;
+    if (g_varh == 1) key = nbr[v] == 0 ? -1 : sz / nbr[v];
+    if (var < 0 || key < best || (planted && g_fault == 2 && key == best)) {
+      best = key;
+      var = v;
+    }
+  }
+  return var;
+}
 
 inline int popc(uint64_t x) { return __builtin_popcountll(x); }
 inline int ctz(uint64_t x) { return __builtin_ctzll(x); }
@@ -500,16 +540,19 @@ uint64_t bf_hash(const BfModel& M, const BfDom& D) {
 struct orc_net {
   std::shared_ptr<Shared> S;
   std::unique_ptr<Net> N;
+  std::vector<int> nbr;  // neighbour counts (Network::neighborhood sizes)
 };
 
 extern "C" {
 
 void orc_set_fault(int fault) { g_fault = fault; }
+void orc_set_var_heuristic(int varh) { g_varh = varh; }
 
 orc_net* orc_create(const cpim_model_desc* desc, int algo) {
   auto* o = new orc_net;
   o->S = build_shared(desc, algo);
   o->N = std::make_unique<Net>(o->S);
+  o->nbr = neighbour_counts(o->S->n, o->S->scope);
   return o;
 }
 
@@ -661,16 +704,8 @@ int orc_mac(orc_net* net, int64_t time_limit_ms, int64_t node_limit, orc_stats* 
         break;
       }
       const int p = (int)I.size();
-      int var = -1;
-      int best = 0;
-      for (int v = 0; v < N.n; ++v) {
-        if (N.assigned(v, p)) continue;
-        const int sz = N.size(v, p);
-        if (var < 0 || sz < best || (g_fault == 2 && sz == best)) {
-          best = sz;
-          var = v;
-        }
-      }
+      const int var = select_var_h(
+          N.n, net->nbr, [&](int v) { return N.size(v, p); }, [&](int v) { return N.assigned(v, p); }, true);
       const int val = N.head(var, p);
       N.new_level(p);
       I.push_back({var, val});
@@ -808,6 +843,7 @@ int orc_bf_mac(const cpim_model_desc* desc, int64_t node_limit, orc_stats* st,
                int32_t* solution, cpim_node_record* trace, int32_t trace_cap,
                int32_t* trace_len) {
   BfModel M = bf_model(desc);
+  const std::vector<int> nbr = neighbour_counts(M.n, M.scope);
   orc_stats s;
   std::memset(&s, 0, sizeof(s));
   int32_t tl = 0;
@@ -836,12 +872,8 @@ int orc_bf_mac(const cpim_model_desc* desc, int64_t node_limit, orc_stats* st,
       s.time_out = 1;
       break;
     }
-    int x = -1, best = 0;
-    for (int v = 0; v < M.n; ++v) {
-      if (decided[v]) continue;
-      const int k = dsize(D, v);
-      if (x < 0 || k < best) { x = v; best = k; }
-    }
+    const int x = select_var_h(
+        M.n, nbr, [&](int v) { return dsize(D, v); }, [&](int v) { return decided[v] != 0; }, false);
     int a = 0;
     while (!D[x][a]) ++a;
     stack.push_back({D, x, a});

@@ -11,7 +11,8 @@
  *   AC3bit ctor + seek_support        src/AC3bit.cpp:3-38
  *   SAC1::enforce                     src/SAC1.cpp:57-103
  *   MAC::enforce / get_solution /
- *     select_var (dom-min) / select_val (min)   src/MAC.cpp:100-165, 256-271, 353-438
+ *     select_var (dom-min, dom/deg) / select_val (min)   src/MAC.cpp:100-165, 256-271, 353-438
+ *   Network::get_neighbor               src/Network.cpp:467-474
  *   AssignedStack push/pop/update     src/Solver.cpp:36-116
  *   var_que (FIFO, no duplicates)     src/Solver.cpp:180-218
  * Part 2 is independent brute force used to pin part 1 (the verify_gac /
@@ -101,6 +102,9 @@ int orc_bf_mac(const cpim_model_desc* desc, int64_t node_limit, orc_stats* st,
  *  3: revise skips the last value of the domain
  *  4: SAC1 does not repeat passes (one pass only) */
 void orc_set_fault(int fault);
+/* Variable ordering of orc_mac and orc_bf_mac, process-wide: 0 = dom-min
+ * (VRH_DOM_MIN), 1 = dom/deg (VRH_DOM_DEG_MIN, src/MAC.cpp:371-385). */
+void orc_set_var_heuristic(int varh);
 
 #ifdef __cplusplus
 }

@@ -6,6 +6,8 @@ under dom-min / min-value ordering, on the reference's fixtures (golden
 files), on seeded random networks checked against the oracle at run time,
 and at full benchmark sizes through size-independent properties.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -321,6 +323,64 @@ def test_random_mac_traces(gpu, case):
         assert list(sol) == list(r.solution)
 
 
+@pytest.mark.parametrize("case", range(len(RANDOM)))
+def test_random_mac_traces_dom_deg(gpu, case):
+    # dom/deg ordering (MAC.cpp:371-384) node for node against the oracle
+    n, d, e, p, pre = RANDOM[case]
+    desc = workload.random_binary(n, d, e, p, 1300 + case, preremove=pre)
+    with oracle.var_heuristic(1):
+        r = oracle.Oracle(desc).mac(node_limit=3000, trace_cap=6000)
+    with GModelAdapter.Build(desc) as m:
+        st, sol, trace = m.Solve(node_limit=3000, trace_capacity=6000, heuristic="dom/deg")
+    assert (st.num_solutions, st.num_positive, st.num_negative, st.time_out) == \
+        (r.num_sol, r.num_positive, r.num_negative, r.time_out)
+    assert trace == r.trace
+    if r.num_sol:
+        assert list(sol) == list(r.solution)
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_golden_mac_dom_deg(gpu, name):
+    desc, g = load_golden(name)
+    limit = int(g["node_limit"]) or 3000
+    with oracle.var_heuristic(1):
+        r = oracle.Oracle(desc).mac(node_limit=limit, trace_cap=2 * limit + 16)
+    with GModelAdapter.Build(desc) as m:
+        st, sol, trace = m.Solve(node_limit=limit, trace_capacity=2 * limit + 16, heuristic="dom/deg")
+    assert (st.num_solutions, st.num_positive, st.num_negative, st.time_out) == \
+        (r.num_sol, r.num_positive, r.num_negative, r.time_out)
+    assert trace == r.trace
+
+
+def test_dom_deg_isolated_variable_on_device(gpu):
+    # a variable in no constraint has key -1 and is decided first
+    dom = [4, 3, 3, 3, 5]
+    scopes, lists = [], []
+    for x, y in [(0, 1), (0, 2), (0, 3), (1, 2)]:
+        scopes.append((x, y))
+        lists.append([(a, b) for a in range(dom[x]) for b in range(dom[y]) if a != b])
+    desc = from_tuples(dom, scopes, lists)
+    with GModelAdapter.Build(desc) as m:
+        st, sol, trace = m.Solve(trace_capacity=64, heuristic="dom/deg")
+    assert [t[1:3] for t in trace] == [(4, 0), (0, 0), (1, 1), (2, 2), (3, 1)]
+    assert st.num_solutions == 1
+
+
+def test_unknown_heuristic_rejected(gpu):
+    desc = workload.random_binary(6, 4, 6, 0.4, 1)
+    with GModelAdapter.Build(desc) as m:
+        with pytest.raises(ValueError):
+            m.Solve(heuristic="wdeg")
+        o = N.CSolveOpts()
+        o.var_heuristic = 7
+        st = N.CSearchStats()
+        sol = np.zeros(6, np.int32)
+        tl = C.c_int32(0)
+        with pytest.raises(N.CpimError, match="heuristic"):
+            N.check(N.gpu_lib().cpim_gmodel_solve(m.handle, C.byref(o), C.byref(st),
+                                                  sol.ctypes.data_as(N.p_i32), None, C.byref(tl)))
+
+
 def test_mac_across_launch_slices(gpu):
     # more nodes than one launch runs: the device state is resumed
     desc = workload.random_binary(23, 23, 253, 0.247, 4)
@@ -359,6 +419,14 @@ def test_tables_gac_and_mac(gpu, seed):
     r = oracle.Oracle(desc, oracle.AC3).mac(node_limit=2000, trace_cap=4000)
     with GModelAdapter.Build(desc, opts) as m:
         st, sol, trace = m.Solve(node_limit=2000, trace_capacity=4000)
+    assert (st.num_solutions, st.num_positive, st.num_negative) == \
+        (r.num_sol, r.num_positive, r.num_negative)
+    assert trace == r.trace
+    # dom/deg counts the neighbours through the tables as well
+    with oracle.var_heuristic(1):
+        r = oracle.Oracle(desc, oracle.AC3).mac(node_limit=2000, trace_cap=4000)
+    with GModelAdapter.Build(desc, opts) as m:
+        st, sol, trace = m.Solve(node_limit=2000, trace_capacity=4000, heuristic="dom/deg")
     assert (st.num_solutions, st.num_positive, st.num_negative) == \
         (r.num_sol, r.num_positive, r.num_negative)
     assert trace == r.trace

@@ -241,3 +241,138 @@ def test_planted_faults_are_caught(fault):
     with oracle.fault(fault):
         caught = not all(pins_hold(c) for c in cases)
     assert caught, f"planted fault {fault} went unnoticed"
+
+
+# ---------------------------------------------------------------------------
+# dom/deg variable ordering (MAC::select_var, src/MAC.cpp:371-384)
+# ---------------------------------------------------------------------------
+def py_mac(desc, varh, node_limit):
+    """A second MAC written from the definitions: GAC as the greatest fixpoint
+    of "a value stays while every constraint on its variable has a tuple
+    through it inside the current domains", neighbours as Python sets, and the
+    binary branching of MAC::enforce (src/MAC.cpp:79-168).  Returns the node
+    sequence (positive, var, value, consistent) and the counts."""
+    n = desc.num_vars
+    scopes = [tuple(int(v) for v in s) for s in desc.scopes()]
+    tuples = [[tuple(int(x) for x in row) for row in t] for t in desc.tuple_lists()]
+    nbrs = [set() for _ in range(n)]
+    for sc in scopes:
+        for x in sc:
+            nbrs[x].update(y for y in sc if y != x)
+
+    def close(D):
+        D = [set(s) for s in D]
+        changed = True
+        while changed:
+            changed = False
+            for sc, tup in zip(scopes, tuples):
+                live = [t for t in tup if all(t[i] in D[v] for i, v in enumerate(sc))]
+                for i, v in enumerate(sc):
+                    keep = {t[i] for t in live}
+                    if keep != D[v]:
+                        D[v] &= keep
+                        changed = True
+                        if not D[v]:
+                            return None
+        return D
+
+    def key(D, v):
+        if varh == 0:
+            return len(D[v])
+        return -1 if not nbrs[v] else len(D[v]) // len(nbrs[v])
+
+    D0 = close([set(s) for s in domains_to_sets(desc.initial_domains(), desc.dom_size)])
+    nodes, pos, neg = [], 0, 0
+    if D0 is None:
+        return nodes, pos, neg, False
+    levels = [D0]          # levels[k] = domains after k decisions
+    stack = []             # (var, val) of the positive decisions
+    while True:
+        if node_limit and pos + neg >= node_limit:
+            return nodes, pos, neg, False
+        D = levels[-1]
+        free = [v for v in range(n) if all(v != x for x, _ in stack)]
+        best = min(key(D, v) for v in free)
+        x = next(v for v in free if key(D, v) == best)
+        a = min(D[x])
+        stack.append((x, a))
+        pos += 1
+        E = [set(s) for s in D]
+        E[x] = {a}
+        E = close(E)
+        nodes.append((1, x, a, E is not None))
+        if E is not None:
+            levels.append(E)
+            if len(stack) == n:
+                return nodes, pos, neg, True
+            continue
+        while True:
+            if not stack:
+                return nodes, pos, neg, False
+            x, a = stack.pop()
+            D = [set(s) for s in levels[len(stack)]]
+            D[x].discard(a)
+            neg += 1
+            E = close(D) if D[x] else None
+            nodes.append((0, x, a, E is not None))
+            # the refuted level replaces the one it came from
+            del levels[len(stack):]
+            if E is not None:
+                levels.append(E)
+                break
+
+
+TINY = [(7, 4, 10, 0.45), (8, 5, 12, 0.5), (6, 6, 9, 0.55), (9, 3, 12, 0.4)]
+
+
+@pytest.mark.parametrize("case", range(len(TINY)))
+@pytest.mark.parametrize("seed", [1, 2])
+@pytest.mark.parametrize("varh", [0, 1], ids=["dom", "domdeg"])
+def test_mac_orderings_vs_python_mac(case, seed, varh):
+    n, d, e, p = TINY[case]
+    desc = workload.random_binary(n, d, e, p, 500 + 10 * case + seed)
+    want, pos, neg, sat = py_mac(desc, varh, node_limit=400)
+    with oracle.var_heuristic(varh):
+        r = oracle.Oracle(desc).mac(node_limit=400, trace_cap=1000)
+        b = oracle.bf_mac(desc, node_limit=400, trace_cap=1000)
+    assert [t[:4] for t in r.trace] == want
+    assert (r.num_positive, r.num_negative, bool(r.num_sol)) == (pos, neg, sat)
+    assert r.trace == b.trace
+
+
+def star_model(extra_isolated=False):
+    """Centre 0 joined to 1, 2, 3 by "not equal"; 1 and 2 also joined.  Sizes
+    4, 3, 3, 3 (+ a size-5 variable in no constraint)."""
+    dom = [4, 3, 3, 3] + ([5] if extra_isolated else [])
+    scopes, lists = [], []
+    for x, y in [(0, 1), (0, 2), (0, 3), (1, 2)]:
+        scopes.append((x, y))
+        lists.append([(a, b) for a in range(dom[x]) for b in range(dom[y]) if a != b])
+    return from_tuples(dom, scopes, lists)
+
+
+def test_dom_deg_star_closed_form():
+    # dom/deg: neighbours 3, 2, 2, 1, keys 4//3=1, 3//2=1, 1, 3//1=3, so 0=0
+    # first (lowest id on the tie); then 1..3 hold {1,2}, keys 1, 1, 2: 1=1;
+    # then 2 holds {2} (key 0): 2=2; last 3=1.
+    # dom-min: 1=0 first (size 3, the first smallest); 0 and 2 lose 0, 2 holds
+    # {1,2}: 2=1; 0 holds {2,3}, 3 holds {0,1,2}: 0=2; last 3=0.
+    desc = star_model()
+    with oracle.var_heuristic(0):
+        dm = oracle.Oracle(desc).mac(trace_cap=64)
+    with oracle.var_heuristic(1):
+        dd = oracle.Oracle(desc).mac(trace_cap=64)
+    assert dm.num_sol == dd.num_sol == 1 and dm.num_negative == dd.num_negative == 0
+    assert [t[1:3] for t in dm.trace] == [(1, 0), (2, 1), (0, 2), (3, 0)]
+    assert [t[1:3] for t in dd.trace] == [(0, 0), (1, 1), (2, 2), (3, 1)]
+    assert check_solution(desc, dd.solution)
+
+
+def test_dom_deg_isolated_variable_first():
+    # a variable with no neighbours has key -1 and is decided first, although
+    # its domain is the largest
+    desc = star_model(extra_isolated=True)
+    with oracle.var_heuristic(1):
+        r = oracle.Oracle(desc).mac(trace_cap=64)
+    assert [t[1] for t in r.trace] == [4, 0, 1, 2, 3]
+    assert r.solution[4] == 0
