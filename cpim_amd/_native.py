@@ -36,6 +36,7 @@ CPIM_SAC_SLICED = 3
 CPIM_BRANCH_BINARY = 0
 CPIM_VARH_DOM_MIN = 0
 CPIM_VARH_DOM_DEG = 1
+CPIM_VARH_DOM_WDEG = 2
 
 p_i32 = C.POINTER(C.c_int32)
 p_i64 = C.POINTER(C.c_int64)

@@ -6,7 +6,7 @@
 // reference's "MAC stats:" line.
 //
 //   cpim_solve --n 30 --d 16 --e 120 --p 0.4 --seed 1 [--node_limit K]
-//              [--time_limit MS] [--varh dom|domdeg] [--sac] [--dway]
+//              [--time_limit MS] [--varh dom|domdeg|domwdeg] [--sac] [--dway]
 //              [--validate-only]
 #include <cstdio>
 #include <cstdlib>
@@ -43,8 +43,9 @@ int main(int argc, char** argv) {
       const std::string h = next();
       if (h == "dom") varh = CPIM_VARH_DOM_MIN;
       else if (h == "domdeg") varh = CPIM_VARH_DOM_DEG;
+      else if (h == "domwdeg") varh = CPIM_VARH_DOM_WDEG;
       else {
-        std::fprintf(stderr, "--varh takes dom or domdeg\n");
+        std::fprintf(stderr, "--varh takes dom, domdeg or domwdeg\n");
         return 2;
       }
     } else if (a == "--sac") sac = true;

@@ -234,7 +234,8 @@ __device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* s
 // tuple indices).  valid = AND over positions of OR over live values of
 // tsup[pos][value]; value b of the variable at position p keeps iff
 // tsup[p][b] AND valid is not empty.  Positions whose variable is `skip_var`,
-// or is assigned (asg bitmap, may be NULL), are not revised.  *wiped is set if this wave
+// or is assigned (asg bitmap, may be NULL), are not revised; with
+// only_var >= 0, only the position of that variable is.  *wiped is set if this wave
 // emptied a domain.  on_removed(var, count) is called (by the whole wave)
 // for every variable that lost values.  Requires tw <= 64 (4096 tuples).
 //
@@ -263,7 +264,7 @@ __device__ __forceinline__ int kth_value(u64 d_l, int W, int k) {
 template <typename OnRemoved, bool kCoherent = false>
 __device__ __forceinline__ void revise_table(const DevNet& net, u64* dom, int* size, int c,
                                              int skip_var, const unsigned* asg,
-                                             OnRemoved on_removed, bool* wiped) {
+                                             OnRemoved on_removed, bool* wiped, int only_var = -1) {
   const int W = net.W;
   const int dpad = net.dpad;
   const int lane = lane_id();
@@ -297,6 +298,7 @@ __device__ __forceinline__ void revise_table(const DevNet& net, u64* dom, int* s
   for (int p = 0; p < arity; ++p) {
     const int y = mp->scope[p];
     if (y == skip_var) continue;
+    if (only_var >= 0 && y != only_var) continue;
     if (asg && ((asg[y >> 5] >> (y & 31)) & 1u)) continue;
     const u64 du_l = lane < W ? ld_dom<kCoherent>(&dom[(long long)y * W + lane]) : 0ull;
     u64 mask_l = 0ull;

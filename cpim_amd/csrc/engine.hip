@@ -1468,7 +1468,8 @@ int cpim_gmodel_solve(cpim_gmodel* m, const cpim_solve_opts* opts, cpim_search_s
     cpim_solve_opts o{};
     if (opts) o = *opts;
     REQUIRE(o.branching == CPIM_BRANCH_BINARY, "the device solver implements binary branching only");
-    REQUIRE(o.var_heuristic == CPIM_VARH_DOM_MIN || o.var_heuristic == CPIM_VARH_DOM_DEG,
+    REQUIRE(o.var_heuristic == CPIM_VARH_DOM_MIN || o.var_heuristic == CPIM_VARH_DOM_DEG ||
+                o.var_heuristic == CPIM_VARH_DOM_WDEG,
             "unknown variable heuristic");
     REQUIRE(o.reserved == 0, "cpim_solve_opts.reserved must be 0");
     REQUIRE(o.trace_capacity >= 0, "negative trace capacity");
@@ -1495,6 +1496,22 @@ int cpim_gmodel_solve(cpim_gmodel* m, const cpim_solve_opts* opts, cpim_search_s
     A.deadline = 0;
     A.root_out = nullptr;
     A.nbr = o.var_heuristic == CPIM_VARH_DOM_DEG ? m->dm.nbr.as<int>() : nullptr;
+    DevAlloc wdeg_weight, wdeg_stamp, wdeg_clock;
+    if (o.var_heuristic == CPIM_VARH_DOM_WDEG) {
+      const int nc = std::max(1, m->dm.num_active);
+      const std::vector<float> ones(nc, 1.0f);
+      const long long one = 1;
+      wdeg_weight = DevAlloc(sizeof(float) * nc);
+      wdeg_stamp = DevAlloc(sizeof(long long) * nc);
+      wdeg_clock = DevAlloc(sizeof(long long));
+      HIPC(hipMemcpyAsync(wdeg_weight.p, ones.data(), sizeof(float) * nc, hipMemcpyHostToDevice, m->stream));
+      HIPC(hipMemsetAsync(wdeg_stamp.p, 0, sizeof(long long) * nc, m->stream));
+      HIPC(hipMemcpyAsync(wdeg_clock.p, &one, sizeof(one), hipMemcpyHostToDevice, m->stream));
+      A.weight = wdeg_weight.as<float>();
+      A.stamp_tab = wdeg_stamp.as<long long>();
+      A.clock = wdeg_clock.as<long long>();
+      HIPC(hipStreamSynchronize(m->stream));  // `ones` and `one` are host stack memory
+    }
     DevNet net = m->dm.net();
     const long long bytes = wg_mem_bytes(n, m->dm.W);
     const int threads = 1024;

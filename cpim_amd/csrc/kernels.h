@@ -465,7 +465,7 @@ __device__ __forceinline__ void wg_mark(unsigned* nxt, WgCtl* ctl, int v, int sl
 // arcs' metadata together, then the wave revises them one by one.
 constexpr int kWgArcs = 8;  // most arcs per work item
 
-__device__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_dirs_round0) {
+__device__ __forceinline__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_dirs_round0) {
   const int n = net.n;
   const int nvw = (n + 31) / 32;
   const int tid = threadIdx.x;
@@ -1436,11 +1436,16 @@ struct MacArgs {
   u64 deadline;           // wall_clock64 ticks (0 = none)
   u64* root_out;          // [n][W] root-fixpoint domains, or NULL
   const int* nbr;         // [n] neighbour counts: dom/deg ordering; NULL = dom-min
+  // dom/wdeg ordering (all NULL otherwise); fixpoints then follow AC3's
+  // schedule so that the weights grow as the reference's do
+  float* weight;          // [ncons] constraint weights, start at 1
+  long long* stamp_tab;   // [ncons] constraint stamps (absolute times), start at 0
+  long long* clock;       // [1] stamp clock, starts at 1
 };
 
 enum { MAC_INIT = 0, MAC_RUNNING = 1, MAC_SAT = 2, MAC_UNSAT = 3, MAC_LIMIT = 4 };
 
-__device__ void wg_trace(const DevNet& net, WgMem mem, WgCtl* ctl, const MacArgs& A, int positive,
+__device__ __forceinline__ void wg_trace(const DevNet& net, WgMem mem, WgCtl* ctl, const MacArgs& A, int positive,
                          int var, int val, bool ok) {
   if (!A.trace) return;
   const int tl = uniform_int(ctl->trace_len);
@@ -1461,7 +1466,180 @@ __device__ void wg_trace(const DevNet& net, WgMem mem, WgCtl* ctl, const MacArgs
   __syncthreads();
 }
 
-__device__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* ctl, const MacArgs& A) {
+// ---------------------------------------------------------------------------
+// AC3 in the reference's schedule (AC3::enforce, src/AC3.cpp:99-141, with the
+// duplicate-free FIFO var_que, src/Solver.cpp:180-218), run by one wave over
+// the workgroup's domains.  The closure equals that of any other schedule;
+// what the order fixes is which constraint empties a domain first, and that
+// constraint's weight is raised (AC3.cpp:122-126): the input of the dom/wdeg
+// ordering.
+//
+// The queue (ring of n + 1), the queued flags and the variable stamps use the
+// workgroup's arc-list arrays (pre, nxt, list), which this schedule does not
+// otherwise need.  Variable stamps restart at 0 on each call, as in the
+// reference.  Constraint stamps stay in global memory as absolute times: a
+// call starts at time base = *clock, reads a stamp s as s - base when
+// s > base and as 0 otherwise (the reference's reset to 0, without touching
+// every constraint), and leaves *clock past its last stamp.
+// Lane 0 does every load and store of the control state and broadcasts what
+// it read, so the same code is correct whether the arrays are in LDS or in
+// global scratch; control values are wave-uniform.
+// seed: one variable, or -1 for all variables in id order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane0_load(const int* p) {
+  int v = 0;
+  if (lane_id() == 0) v = *p;
+  return __builtin_amdgcn_readlane(v, 0);
+}
+
+__device__ __forceinline__ bool wave_ac3(const DevNet& net, WgMem mem, const MacArgs& A, int seed,
+                         unsigned long long* revs) {
+  const int n = net.n;
+  const int lane = lane_id();
+  const int nvw = (n + 31) / 32;
+  const int qcap = n + 1;
+  int* q = mem.pre;
+  unsigned* inq = mem.nxt;
+  int* sv = mem.list;
+  long long base = 0;
+  if (lane == 0) base = *A.clock;
+  base = (long long)readlane64((u64)base, 0);
+  int front = 0, rear = 0, t = 0;
+  if (seed >= 0) {
+    for (int v = lane; v < n; v += 64) sv[v] = 0;
+    for (int i = lane; i < nvw; i += 64) inq[i] = 0u;
+  } else {
+    for (int v = lane; v < n; v += 64) {
+      q[v] = v;
+      sv[v] = v + 1;
+    }
+    for (int i = lane; i < nvw; i += 64) {
+      const int k = n - 32 * i;
+      inq[i] = k >= 32 ? 0xffffffffu : ((1u << k) - 1u);
+    }
+    rear = n;
+    t = n;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  auto insert = [&](int v) {
+    const int word = lane0_load((const int*)&inq[v >> 5]);
+    if (!((word >> (v & 31)) & 1)) {
+      if (lane == 0) {
+        q[rear] = v;
+        inq[v >> 5] = (unsigned)word | (1u << (v & 31));
+      }
+      rear = rear + 1 == qcap ? 0 : rear + 1;
+    }
+    ++t;
+    if (lane == 0) sv[v] = t;
+  };
+  if (seed >= 0) insert(seed);
+  bool ok = true;
+  unsigned long long nrev = 0;
+  while (ok && front != rear) {
+    const int x = lane0_load(&q[front]);
+    front = front + 1 == qcap ? 0 : front + 1;
+    if (lane == 0) inq[x >> 5] &= ~(1u << (x & 31));
+    const int k1 = mem.aoff[x + 1];
+    for (int k = mem.aoff[x]; k < k1; ++k) {
+      const int c = uniform_int(mem.adj[k].x);
+      long long sc = 0;
+      if (lane == 0) sc = A.stamp_tab[c];
+      sc = (long long)readlane64((u64)sc, 0);
+      const int stc = sc > base ? (int)(sc - base) : 0;
+      if (!(lane0_load(&sv[x]) > stc)) continue;
+      const ConMeta* mp = net.con + c;
+      const int arity = mp->arity;
+      for (int i = 0; i < arity; ++i) {
+        const int y = mp->scope[i];
+        if (bit_of(mem.asg, y)) continue;
+        bool aa = false;
+        for (int j = 0; j < arity; ++j) {
+          const int z = mp->scope[j];
+          if (z != x && lane0_load(&sv[z]) > stc) aa = true;
+        }
+        if (y == x && !aa) continue;
+        bool wiped = false;
+        int removed = 0;
+        if (mp->kind == 0) {
+          removed = revise_binary_at(net, mem.dom, mem.size, mp->off, y, mp->scope[1 - i], 1 - i, &wiped);
+        } else {
+          revise_table(net, mem.dom, mem.size, c, -1, nullptr, [&](int, int r) { removed += r; }, &wiped, y);
+        }
+        ++nrev;
+        if (removed > 0) {
+          if (wiped) {
+            if (lane == 0) A.weight[c] += 1.0f;
+            ok = false;
+            break;
+          }
+          insert(y);
+        }
+      }
+      if (!ok) break;
+      ++t;
+      if (lane == 0) A.stamp_tab[c] = base + t;
+    }
+  }
+  if (lane == 0) *A.clock = base + t + 1;
+  *revs += nrev;
+  return ok;
+}
+
+// AC3-scheduled fixpoint for the whole workgroup: wave 0 runs it, the other
+// waves wait at the barrier.
+__device__ __forceinline__ bool wg_ac3(const DevNet& net, WgMem mem, WgCtl* ctl, const MacArgs& A, int seed) {
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    unsigned long long revs = 0;
+    const bool ok = wave_ac3(net, mem, A, seed, &revs);
+    if (threadIdx.x == 0) {
+      ctl->ok = ok ? 1 : 0;
+      ctl->revs += revs;
+    }
+  }
+  __syncthreads();
+  return uniform_int(ctl->ok) != 0;
+}
+
+// MAC::select_var VRH_DOM_WDEG_MIN (src/MAC.cpp:386-411): for each undecided
+// x, x_w = the sum of the weights of the constraints on x that have another
+// undecided variable; key = -1 when |D(x)| = 1 or x_w = 0, else |D(x)| / x_w
+// in double (IEEE division, as on the host); the lowest id among the smallest
+// keys.  Non-negative doubles order as their bit patterns, -1 maps to 0.
+__device__ __forceinline__ int wg_select_wdeg(const DevNet& net, WgMem mem, WgCtl* ctl, const MacArgs& A) {
+  const int n = net.n;
+  u64 best = ~0ull;
+  int bv = n;
+  for (int v = threadIdx.x; v < n; v += blockDim.x) {
+    if (bit_of(mem.asg, v)) continue;
+    double xw = 0.0;
+    const int k1 = mem.aoff[v + 1];
+    for (int k = mem.aoff[v]; k < k1; ++k) {
+      const int4 e = mem.adj[k];
+      int cnt = 0;
+      if (e.y >= 0) {
+        cnt = 1 + (bit_of(mem.asg, e.y) ? 0 : 1);
+      } else {
+        const ConMeta* mp = net.con + e.x;
+        for (int i = 0; i < mp->arity; ++i) cnt += bit_of(mem.asg, mp->scope[i]) ? 0 : 1;
+      }
+      if (cnt > 1) xw += (double)A.weight[e.x];
+    }
+    const int sz = mem.size[v];
+    const double key = (sz == 1 || xw == 0.0) ? -1.0 : (double)sz / xw;
+    const u64 k = key < 0.0 ? 0ull : (u64)__double_as_longlong(key);
+    if (k < best) {
+      best = k;
+      bv = v;
+    }
+  }
+  const u64 m = wg_reduce_min_u64(ctl, best);
+  return (int)wg_reduce_min_u64(ctl, best == m ? (u64)bv : ~0ull);
+}
+
+__device__ __forceinline__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* ctl, const MacArgs& A) {
   const int n = net.n, W = net.W;
   const int nvw = (n + 31) / 32;
   const long long nw = (long long)n * W;
@@ -1488,7 +1666,7 @@ __device__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* ctl, const MacAr
     wg_recount(mem, n, W);
     wg_fill_bitmap(mem.cur, n);
     __syncthreads();
-    const bool ok = wg_fixpoint(net, mem, ctl, false);
+    const bool ok = A.weight ? wg_ac3(net, mem, ctl, A, -1) : wg_fixpoint(net, mem, ctl, false);
     if (ok) wg_copy_words(A.trail, mem.dom, nw);
     if (A.root_out) wg_copy_words(A.root_out, mem.dom, nw);
     status = ok ? MAC_RUNNING : MAC_UNSAT;
@@ -1516,19 +1694,24 @@ __device__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* ctl, const MacAr
     // select: min key over undecided variables, lowest id on ties.  dom-min:
     // key = |D|; dom/deg: |D| / neighbours (integer), -1 without neighbours,
     // biased by one so that the packed key stays unsigned.
-    u64 key = ~0ull;
-    for (int v = threadIdx.x; v < n; v += blockDim.x)
-      if (!bit_of(mem.asg, v)) {
-        unsigned hi = (unsigned)mem.size[v];
-        if (A.nbr) {
-          const int g = A.nbr[v];
-          hi = g == 0 ? 0u : (unsigned)(mem.size[v] / g) + 1u;
+    int x;
+    if (A.weight) {
+      x = wg_select_wdeg(net, mem, ctl, A);
+    } else {
+      u64 key = ~0ull;
+      for (int v = threadIdx.x; v < n; v += blockDim.x)
+        if (!bit_of(mem.asg, v)) {
+          unsigned hi = (unsigned)mem.size[v];
+          if (A.nbr) {
+            const int g = A.nbr[v];
+            hi = g == 0 ? 0u : (unsigned)(mem.size[v] / g) + 1u;
+          }
+          const u64 k = ((u64)hi << 32) | (unsigned)v;
+          key = k < key ? k : key;
         }
-        const u64 k = ((u64)hi << 32) | (unsigned)v;
-        key = k < key ? k : key;
-      }
-    key = wg_reduce_min_u64(ctl, key);
-    const int x = (int)(key & 0xffffffffu);
+      key = wg_reduce_min_u64(ctl, key);
+      x = (int)(key & 0xffffffffu);
+    }
     // min value of x (every thread scans the same LDS words)
     int a = -1;
     for (int w = 0; w < W && a < 0; ++w) {
@@ -1551,7 +1734,7 @@ __device__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* ctl, const MacAr
     ++npos;
     ++nodes_here;
     __syncthreads();
-    bool ok = wg_fixpoint(net, mem, ctl, false);
+    bool ok = A.weight ? wg_ac3(net, mem, ctl, A, x) : wg_fixpoint(net, mem, ctl, false);
     wg_trace(net, mem, ctl, A, 1, x, a, ok);
     if (ok && depth == n) {
       if (A.solution)
@@ -1589,7 +1772,7 @@ __device__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* ctl, const MacAr
       } else {
         if (threadIdx.x == 0) mem.cur[dx >> 5] |= 1u << (dx & 31);
         __syncthreads();
-        ok = wg_fixpoint(net, mem, ctl, false);
+        ok = A.weight ? wg_ac3(net, mem, ctl, A, dx) : wg_fixpoint(net, mem, ctl, false);
       }
       wg_trace(net, mem, ctl, A, 0, dx, dv, ok);
     }

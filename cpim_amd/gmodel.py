@@ -28,7 +28,8 @@ _SAC_MODES = {"auto": N.CPIM_SAC_BATCHED, "plain": N.CPIM_SAC_PLAIN,
               "workgroup": N.CPIM_SAC_WORKGROUP, "sliced": N.CPIM_SAC_SLICED}
 
 
-_VAR_HEURISTICS = {"dom": N.CPIM_VARH_DOM_MIN, "dom/deg": N.CPIM_VARH_DOM_DEG}
+_VAR_HEURISTICS = {"dom": N.CPIM_VARH_DOM_MIN, "dom/deg": N.CPIM_VARH_DOM_DEG,
+                   "dom/wdeg": N.CPIM_VARH_DOM_WDEG}
 
 
 def _var_heuristic(name: str) -> int:
@@ -274,8 +275,10 @@ class GModel:
               heuristic: str = "dom"):
         """Device MAC (binary branching, CPU MAC semantics) from level 0.
 
-        heuristic: "dom" (smallest domain, MAC.cpp:356-364) or "dom/deg"
-        (domain size over neighbour count, MAC.cpp:371-384)."""
+        heuristic: "dom" (smallest domain, MAC.cpp:356-364), "dom/deg"
+        (domain size over neighbour count, MAC.cpp:371-384) or "dom/wdeg"
+        (domain size over failure-weighted degree, MAC.cpp:386-411; its
+        fixpoints run in AC3's order on one wave)."""
         o = N.CSolveOpts()
         o.time_limit_ms = time_limit_ms
         o.node_limit = node_limit

@@ -1,7 +1,7 @@
 """Command-line solver over the device engine, for XCSP 2.x instance files.
 
     python -m cpim_amd.solve FILE.xml [--time-limit MS] [--node-limit N]
-                             [--varh dom|dom/deg] [--sac] [--dway] [--gac-only]
+                             [--varh dom|dom/deg|dom/wdeg] [--sac] [--dway] [--gac-only]
 
 Prints, like the reference's cpim_test_parser (samples/main_new_parser.cpp:
 200-243), the model size, the solution in canonical indices and in original
@@ -30,8 +30,8 @@ def main(argv=None) -> int:
     ap.add_argument("file")
     ap.add_argument("--time-limit", type=int, default=0, help="ms, 0 = none")
     ap.add_argument("--node-limit", type=int, default=0, help="search nodes, 0 = none")
-    ap.add_argument("--varh", choices=["dom", "dom/deg"], default="dom",
-                    help="variable ordering of the binary search (MAC.cpp:353-385)")
+    ap.add_argument("--varh", choices=["dom", "dom/deg", "dom/wdeg"], default="dom",
+                    help="variable ordering of the binary search (MAC.cpp:353-411)")
     ap.add_argument("--sac", action="store_true", help="enforce SAC before the search")
     ap.add_argument("--dway", action="store_true", help="d-way search of the reference GModelSolver")
     ap.add_argument("--gac-only", action="store_true", help="print the root GAC domains and stop")

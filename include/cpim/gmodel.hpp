@@ -222,10 +222,10 @@ class GModelSolver {
     if (!model_) throw std::runtime_error("[GModelSolver] model is null!");
   }
 
-  // CPIM_VARH_DOM_MIN (default) or CPIM_VARH_DOM_DEG; the d-way search
-  // orders by smallest domain only.
+  // CPIM_VARH_DOM_MIN (default), CPIM_VARH_DOM_DEG or CPIM_VARH_DOM_WDEG;
+  // the d-way search orders by smallest domain only.
   void SetVarHeuristic(int h) {
-    if (h != CPIM_VARH_DOM_MIN && h != CPIM_VARH_DOM_DEG)
+    if (h != CPIM_VARH_DOM_MIN && h != CPIM_VARH_DOM_DEG && h != CPIM_VARH_DOM_WDEG)
       throw std::invalid_argument("unknown variable heuristic");
     if (h != CPIM_VARH_DOM_MIN && branching_ == Branching::kDway)
       throw std::invalid_argument("the d-way search orders variables by smallest domain only");

@@ -239,23 +239,31 @@ int cpim_gmodel_singleton_tests(cpim_gmodel* model, const int32_t* tests, int64_
 
 #define CPIM_BRANCH_BINARY 0 /* CPU MAC: x = a / x != a, dom-min, min value */
 
-/* Variable ordering, MAC::select_var (src/MAC.cpp:353-385): the undecided
+/* Variable ordering, MAC::select_var (src/MAC.cpp:353-411): the undecided
  * variable with the smallest key, the lowest id on ties.
- *   DOM_MIN  key = |D(x)|                         (Heuristic::VRH_DOM_MIN)
- *   DOM_DEG  key = |D(x)| / |neighbours(x)|, in integer division, or -1 for
- *            a variable that shares no constraint  (Heuristic::VRH_DOM_DEG_MIN)
+ *   DOM_MIN   key = |D(x)|                        (Heuristic::VRH_DOM_MIN)
+ *   DOM_DEG   key = |D(x)| / |neighbours(x)|, in integer division, or -1 for
+ *             a variable that shares no constraint (Heuristic::VRH_DOM_DEG_MIN)
+ *   DOM_WDEG  key = |D(x)| / x_w in double, x_w = the summed weights of the
+ *             constraints on x with another undecided variable; -1 when
+ *             |D(x)| = 1 or x_w = 0 (Heuristic::VRH_DOM_WDEG_MIN).  Weights
+ *             start at 1 on every solve and grow by one on the constraint
+ *             that empties a domain first in AC3's order (src/AC3.cpp:122-126);
+ *             with this ordering every fixpoint of the search runs in that
+ *             order, on one wave, so it is slower than the other two.
  * Neighbours are the distinct other variables of the constraints on x
  * (Network::get_neighbor, src/Network.cpp:467-474), counted over the
  * constraints the model propagates. */
 #define CPIM_VARH_DOM_MIN 0
 #define CPIM_VARH_DOM_DEG 1
+#define CPIM_VARH_DOM_WDEG 2
 
 typedef struct cpim_solve_opts {
   int64_t time_limit_ms; /* 0 = unlimited */
   int64_t node_limit;    /* 0 = unlimited; checked where the time limit is */
   int32_t branching;     /* CPIM_BRANCH_BINARY */
   int32_t trace_capacity;/* records kept in trace (0 = none) */
-  int32_t var_heuristic; /* CPIM_VARH_DOM_MIN (0) or CPIM_VARH_DOM_DEG */
+  int32_t var_heuristic; /* CPIM_VARH_DOM_MIN (0), _DOM_DEG or _DOM_WDEG */
   int32_t reserved;      /* 0 */
 } cpim_solve_opts;
 

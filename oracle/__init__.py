@@ -65,6 +65,7 @@ def lib():
         L.orc_destroy.argtypes = [vp]
         L.orc_num_vars.argtypes = [vp]
         L.orc_dom_words.argtypes = [vp]
+        L.orc_weights.argtypes = [vp, C.POINTER(C.c_float)]
         L.orc_root_ac.argtypes = [vp, N.p_u64, C.POINTER(OrcStats)]
         L.orc_sac1.argtypes = [vp, N.p_u64, C.POINTER(OrcStats)]
         L.orc_singleton_tests.argtypes = [vp, N.p_i32, C.c_int64, N.p_u8, C.c_int, C.POINTER(OrcStats)]
@@ -140,6 +141,13 @@ class Oracle:
                                   nthreads, C.byref(st))
         return fail, st
 
+    def weights(self) -> np.ndarray:
+        """Constraint weights (raised by AC3 on each wipe-out)."""
+        k = lib().orc_weights(self.h, None)
+        w = np.zeros(max(1, k), np.float32)
+        lib().orc_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_float)))
+        return w[:k]
+
     def mac(self, time_limit_ms: int = 0, node_limit: int = 0, trace_cap: int = 0) -> MacResult:
         sol = np.full(self.n, -1, np.int32)
         tr = (N.CNodeRecord * max(1, trace_cap))()
@@ -185,8 +193,10 @@ def bf_mac(desc, node_limit: int = 0, trace_cap: int = 0) -> MacResult:
     tr = (N.CNodeRecord * max(1, trace_cap))()
     tl = C.c_int32(0)
     st = OrcStats()
-    lib().orc_bf_mac(C.byref(c), node_limit, C.byref(st), _ptr(sol, C.c_int32), tr, trace_cap,
-                     C.byref(tl))
+    rc = lib().orc_bf_mac(C.byref(c), node_limit, C.byref(st), _ptr(sol, C.c_int32), tr,
+                          trace_cap, C.byref(tl))
+    if rc < 0:
+        raise ValueError("the brute-force MAC has no dom/wdeg form")
     trace = [(r.positive, r.var, r.value, r.consistent, r.dom_hash)
              for r in tr[:min(tl.value, trace_cap)]]
     return MacResult(int(st.num_sol), int(st.num_positive), int(st.num_negative),
@@ -195,7 +205,8 @@ def bf_mac(desc, node_limit: int = 0, trace_cap: int = 0) -> MacResult:
 
 class var_heuristic:
     """Context manager selecting the oracle's variable ordering
-    (0 = dom-min, 1 = dom/deg) for orc_mac and orc_bf_mac."""
+    (0 = dom-min, 1 = dom/deg, 2 = dom/wdeg) for orc_mac and orc_bf_mac
+    (which has no dom/wdeg form)."""
 
     def __init__(self, varh: int):
         self.varh = varh

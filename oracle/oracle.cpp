@@ -13,7 +13,7 @@
 namespace {
 
 int g_fault = 0;
-int g_varh = 0;  // 0 = dom-min (VRH_DOM_MIN), 1 = dom/deg (VRH_DOM_DEG_MIN)
+int g_varh = 0;  // 0 = dom-min (VRH_DOM_MIN), 1 = dom/deg (VRH_DOM_DEG_MIN), 2 = dom/wdeg
 
 // Number of distinct other variables sharing a constraint with each variable
 // (Network::get_neighbor, src/Network.cpp:467-474).
@@ -387,7 +387,7 @@ struct AC3 {
             if ((y != x) || aa) {
               if (revise(c, y, level)) {
                 if (N.size(y, level) == 0) {
-                  N.weight[c] += 1.0f;
+                  if (g_fault != 5) N.weight[c] += 1.0f;
                   return false;
                 }
                 insert(y);
@@ -402,6 +402,33 @@ struct AC3 {
     return true;
   }
 };
+
+// MAC::select_var VRH_DOM_WDEG_MIN (src/MAC.cpp:386-411): for each
+// undecided x, x_w = sum of the weights of the constraints on x with more
+// than one undecided variable; key = -1 when |D(x)| = 1 or x_w = 0, else
+// |D(x)| / x_w in double; the first variable with the smallest key.  The
+// weights are the Tabular::weight counts that AC3 raises on a wipe-out.
+int select_var_wdeg(const Net& N, int p) {
+  int var = -1;
+  double best = 1.7976931348623157e308;  // DBL_MAX
+  for (int x = 0; x < N.n; ++x) {
+    if (N.assigned(x, p)) continue;
+    double xw = 0.0;
+    for (int c : N.S->subscription[x]) {
+      int cnt = 0;
+      for (int y : N.S->scope[c])
+        if (!N.assigned(y, p)) ++cnt;
+      if (cnt > 1) xw += N.weight[c];
+    }
+    const int sz = N.size(x, p);
+    const double key = (sz == 1 || xw == 0.0) ? -1.0 : sz / xw;
+    if (key < best) {
+      best = key;
+      var = x;
+    }
+  }
+  return var;
+}
 
 std::vector<int> all_vars(int n) {
   std::vector<int> v(n);
@@ -559,6 +586,11 @@ orc_net* orc_create(const cpim_model_desc* desc, int algo) {
 void orc_destroy(orc_net* net) { delete net; }
 int orc_num_vars(const orc_net* net) { return net->S->n; }
 int orc_dom_words(const orc_net* net) { return net->S->Wm; }
+int orc_weights(const orc_net* net, float* out) {
+  const auto& w = net->N->weight;
+  if (out) std::copy(w.begin(), w.end(), out);
+  return (int)w.size();
+}
 
 static void copy_out(const Net& N, int level, uint64_t* out) {
   if (!out) return;
@@ -665,8 +697,11 @@ int orc_singleton_tests(orc_net* net, const int32_t* tests, int64_t count, uint8
   return 0;
 }
 
-// MAC::enforce (MAC.cpp:100-165) with select_var VRH_DOM_MIN (:357-364) and
-// select_val VLH_MIN (:423-425); AssignedStack (Solver.cpp:36-116).
+// MAC::enforce (MAC.cpp:100-165) with select_var VRH_DOM_MIN (:357-364),
+// VRH_DOM_DEG_MIN (:371-384) or VRH_DOM_WDEG_MIN (:386-411) as g_varh says,
+// select_val VLH_MIN (:423-425); AssignedStack (Solver.cpp:36-116).  The
+// constraint weights live in the network and persist across calls, as
+// Tabular::weight does.
 int orc_mac(orc_net* net, int64_t time_limit_ms, int64_t node_limit, orc_stats* st,
             int32_t* solution, cpim_node_record* trace, int32_t trace_cap, int32_t* trace_len) {
   Net& N = *net->N;
@@ -704,8 +739,10 @@ int orc_mac(orc_net* net, int64_t time_limit_ms, int64_t node_limit, orc_stats* 
         break;
       }
       const int p = (int)I.size();
-      const int var = select_var_h(
-          N.n, net->nbr, [&](int v) { return N.size(v, p); }, [&](int v) { return N.assigned(v, p); }, true);
+      const int var = g_varh == 2 ? select_var_wdeg(N, p)
+                                  : select_var_h(
+                                        N.n, net->nbr, [&](int v) { return N.size(v, p); },
+                                        [&](int v) { return N.assigned(v, p); }, true);
       const int val = N.head(var, p);
       N.new_level(p);
       I.push_back({var, val});
@@ -842,6 +879,8 @@ int64_t orc_bf_count_solutions(const cpim_model_desc* desc, int64_t limit, int32
 int orc_bf_mac(const cpim_model_desc* desc, int64_t node_limit, orc_stats* st,
                int32_t* solution, cpim_node_record* trace, int32_t trace_cap,
                int32_t* trace_len) {
+  // dom/wdeg has no brute-force counterpart: its weights follow AC3's order
+  if (g_varh == 2) return -1;
   BfModel M = bf_model(desc);
   const std::vector<int> nbr = neighbour_counts(M.n, M.scope);
   orc_stats s;

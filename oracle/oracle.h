@@ -57,6 +57,9 @@ orc_net* orc_create(const cpim_model_desc* desc, int algo);
 void orc_destroy(orc_net* net);
 int orc_num_vars(const orc_net* net);
 int orc_dom_words(const orc_net* net);
+/* Tabular::weight of every propagated constraint (wdeg bookkeeping) into out
+ * (may be NULL); returns the number of constraints. */
+int orc_weights(const orc_net* net, float* out);
 
 /* Root AC (all variables queued, level 0).  Domains of level 0 are written
  * to out ([n][dom_words]) when out != NULL.  Returns 1 if consistent. */
@@ -100,10 +103,13 @@ int orc_bf_mac(const cpim_model_desc* desc, int64_t node_limit, orc_stats* st,
  *  1: AC3bit seek_support ignores the last word of the other domain
  *  2: dom-min breaks ties towards the highest variable id
  *  3: revise skips the last value of the domain
- *  4: SAC1 does not repeat passes (one pass only) */
+ *  4: SAC1 does not repeat passes (one pass only)
+ *  5: AC3 does not raise the weight of the constraint that wiped a domain */
 void orc_set_fault(int fault);
 /* Variable ordering of orc_mac and orc_bf_mac, process-wide: 0 = dom-min
- * (VRH_DOM_MIN), 1 = dom/deg (VRH_DOM_DEG_MIN, src/MAC.cpp:371-385). */
+ * (VRH_DOM_MIN), 1 = dom/deg (VRH_DOM_DEG_MIN, src/MAC.cpp:371-385),
+ * 2 = dom/wdeg (VRH_DOM_WDEG_MIN, src/MAC.cpp:386-411, weights raised by AC3
+ * on a wipe-out, src/AC3.cpp:122-129; orc_bf_mac returns -1 for it). */
 void orc_set_var_heuristic(int varh);
 
 #ifdef __cplusplus

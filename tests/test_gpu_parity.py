@@ -352,6 +352,46 @@ def test_golden_mac_dom_deg(gpu, name):
     assert trace == r.trace
 
 
+@pytest.mark.parametrize("case", range(len(RANDOM)))
+def test_random_mac_traces_dom_wdeg(gpu, case):
+    # dom/wdeg (MAC.cpp:386-411): fixpoints in AC3's order, weights raised on
+    # each wipe-out; node for node and weight for weight against the oracle
+    n, d, e, p, pre = RANDOM[case]
+    desc = workload.random_binary(n, d, e, p, 1300 + case, preremove=pre)
+    with oracle.var_heuristic(2):
+        r = oracle.Oracle(desc).mac(node_limit=3000, trace_cap=6000)
+    with GModelAdapter.Build(desc) as m:
+        st, sol, trace = m.Solve(node_limit=3000, trace_capacity=6000, heuristic="dom/wdeg")
+    assert (st.num_solutions, st.num_positive, st.num_negative, st.time_out) == \
+        (r.num_sol, r.num_positive, r.num_negative, r.time_out)
+    assert trace == r.trace
+    if r.num_sol:
+        assert list(sol) == list(r.solution)
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_golden_mac_dom_wdeg(gpu, name):
+    desc, g = load_golden(name)
+    limit = int(g["node_limit"]) or 3000
+    with oracle.var_heuristic(2):
+        r = oracle.Oracle(desc).mac(node_limit=limit, trace_cap=2 * limit + 16)
+    with GModelAdapter.Build(desc) as m:
+        st, sol, trace = m.Solve(node_limit=limit, trace_capacity=2 * limit + 16, heuristic="dom/wdeg")
+    assert (st.num_solutions, st.num_positive, st.num_negative, st.time_out) == \
+        (r.num_sol, r.num_positive, r.num_negative, r.time_out)
+    assert trace == r.trace
+
+
+def test_dom_wdeg_triangle_on_device(gpu):
+    # 0 != 1, 1 != 2, 0 != 2 over {0, 1}: 0=0 fails, its refutation fails
+    desc = from_tuples([2, 2, 2], [(0, 1), (1, 2), (0, 2)],
+                       [[(a, b) for a in range(2) for b in range(2) if a != b]] * 3)
+    with GModelAdapter.Build(desc) as m:
+        st, sol, trace = m.Solve(trace_capacity=16, heuristic="dom/wdeg")
+    assert [t[:4] for t in trace] == [(1, 0, 0, 0), (0, 0, 0, 0)]
+    assert (st.num_solutions, st.num_positive, st.num_negative) == (0, 1, 1)
+
+
 def test_dom_deg_isolated_variable_on_device(gpu):
     # a variable in no constraint has key -1 and is decided first
     dom = [4, 3, 3, 3, 5]
@@ -422,14 +462,16 @@ def test_tables_gac_and_mac(gpu, seed):
     assert (st.num_solutions, st.num_positive, st.num_negative) == \
         (r.num_sol, r.num_positive, r.num_negative)
     assert trace == r.trace
-    # dom/deg counts the neighbours through the tables as well
-    with oracle.var_heuristic(1):
-        r = oracle.Oracle(desc, oracle.AC3).mac(node_limit=2000, trace_cap=4000)
-    with GModelAdapter.Build(desc, opts) as m:
-        st, sol, trace = m.Solve(node_limit=2000, trace_capacity=4000, heuristic="dom/deg")
-    assert (st.num_solutions, st.num_positive, st.num_negative) == \
-        (r.num_sol, r.num_positive, r.num_negative)
-    assert trace == r.trace
+    # dom/deg counts the neighbours through the tables as well; dom/wdeg
+    # revises one table position at a time in AC3's order
+    for varh, name in ((1, "dom/deg"), (2, "dom/wdeg")):
+        with oracle.var_heuristic(varh):
+            r = oracle.Oracle(desc, oracle.AC3).mac(node_limit=2000, trace_cap=4000)
+        with GModelAdapter.Build(desc, opts) as m:
+            st, sol, trace = m.Solve(node_limit=2000, trace_capacity=4000, heuristic=name)
+        assert (st.num_solutions, st.num_positive, st.num_negative) == \
+            (r.num_sol, r.num_positive, r.num_negative), name
+        assert trace == r.trace, name
 
 
 # ---------------------------------------------------------------------------

@@ -376,3 +376,203 @@ def test_dom_deg_isolated_variable_first():
         r = oracle.Oracle(desc).mac(trace_cap=64)
     assert [t[1] for t in r.trace] == [4, 0, 1, 2, 3]
     assert r.solution[4] == 0
+
+
+# ---------------------------------------------------------------------------
+# dom/wdeg ordering (MAC.cpp:386-411) and AC3's weight bookkeeping
+# (AC3.cpp:99-141: variable queue, time stamps, weight raised on a wipe-out)
+# ---------------------------------------------------------------------------
+def py_mac_wdeg(desc, node_limit):
+    """A second MAC under dom/wdeg, written in Python from the reference's
+    definitions: AC3 with the duplicate-free FIFO variable queue and the
+    variable / constraint time stamps, revise by tuple enumeration, the
+    weight of the constraint whose revision empties a domain raised by one,
+    and the dom/wdeg key.  Returns the node sequence (positive, var, value,
+    consistent), the counts and the final weights."""
+    from collections import deque
+    n = desc.num_vars
+    scopes = [tuple(int(v) for v in s) for s in desc.scopes()]
+    tuples = [[tuple(int(x) for x in row) for row in t] for t in desc.tuple_lists()]
+    subs = [[] for _ in range(n)]
+    for c, sc in enumerate(scopes):
+        for v in sc:
+            subs[v].append(c)
+    weight = [1.0] * len(scopes)
+
+    def ac3(D, evt, assigned):
+        sv, st = [0] * n, [0] * len(scopes)
+        t = 0
+        q, inq = deque(), [False] * n
+
+        def insert(v):
+            nonlocal t
+            if not inq[v]:
+                q.append(v)
+                inq[v] = True
+            t += 1
+            sv[v] = t
+
+        for v in evt:
+            insert(v)
+        while q:
+            x = q.popleft()
+            inq[x] = False
+            for c in subs[x]:
+                if not sv[x] > st[c]:
+                    continue
+                sc = scopes[c]
+                for i, y in enumerate(sc):
+                    if y in assigned:
+                        continue
+                    aa = any(z != x and sv[z] > st[c] for z in sc)
+                    if y == x and not aa:
+                        continue
+                    keep = {tp[i] for tp in tuples[c]
+                            if all(tp[j] in D[sc[j]] for j in range(len(sc)))}
+                    if D[y] - keep:
+                        D[y] = D[y] & keep
+                        if not D[y]:
+                            weight[c] += 1.0
+                            return False
+                        insert(y)
+                t += 1
+                st[c] = t
+        return True
+
+    def key(D, v, assigned):
+        xw = 0.0
+        for c in subs[v]:
+            if sum(1 for y in scopes[c] if y not in assigned) > 1:
+                xw += weight[c]
+        return -1.0 if (len(D[v]) == 1 or xw == 0.0) else len(D[v]) / xw
+
+    D = [set(s) for s in domains_to_sets(desc.initial_domains(), desc.dom_size)]
+    nodes, pos, neg = [], 0, 0
+    if not ac3(D, list(range(n)), set()):
+        return nodes, pos, neg, False, weight
+    levels, stack = [D], []
+    while True:
+        if node_limit and pos + neg >= node_limit:
+            return nodes, pos, neg, False, weight
+        D = levels[-1]
+        assigned = {v for v, _ in stack}
+        free = [v for v in range(n) if v not in assigned]
+        keys = [key(D, v, assigned) for v in free]
+        x = free[keys.index(min(keys))]
+        a = min(D[x])
+        stack.append((x, a))
+        E = [set(s) for s in D]
+        E[x] = {a}
+        ok = ac3(E, [x], {v for v, _ in stack})
+        pos += 1
+        nodes.append((1, x, a, ok))
+        if ok:
+            levels.append(E)
+            if len(stack) == n:
+                return nodes, pos, neg, True, weight
+            continue
+        while True:
+            if not stack:
+                return nodes, pos, neg, False, weight
+            x, a = stack.pop()
+            E = [set(s) for s in levels[len(stack)]]
+            E[x].discard(a)
+            neg += 1
+            ok = bool(E[x]) and ac3(E, [x], {v for v, _ in stack})
+            nodes.append((0, x, a, ok))
+            del levels[len(stack):]
+            if ok:
+                levels.append(E)
+                break
+
+
+# (n, d, e, p), seed: satisfiable and unsatisfiable cases with failures (and
+# so weight changes) early in the search
+WDEG = [((16, 4, 50, 0.3), 901), ((14, 6, 35, 0.5), 903), ((12, 8, 25, 0.6), 902),
+        ((25, 6, 80, 0.42), 903), ((30, 6, 100, 0.4), 901), ((30, 6, 100, 0.4), 902),
+        ((18, 6, 40, 0.4), 903), ((20, 5, 50, 0.33), 901), ((11, 6, 20, 0.45), 901),
+        ((8, 5, 12, 0.5), 713)]
+
+
+@pytest.mark.parametrize("case", range(len(WDEG)))
+def test_mac_wdeg_vs_python_ac3(case):
+    cfg, seed = WDEG[case]
+    desc = workload.random_binary(*cfg, seed)
+    want, pos, neg, sat, weight = py_mac_wdeg(desc, node_limit=300)
+    o = oracle.Oracle(desc)
+    with oracle.var_heuristic(2):
+        r = o.mac(node_limit=300, trace_cap=1000)
+    assert [t[:4] for t in r.trace] == want
+    assert (r.num_positive, r.num_negative, bool(r.num_sol)) == (pos, neg, sat)
+    assert o.weights().tolist() == weight
+    if r.num_sol:
+        assert check_solution(desc, r.solution)
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_mac_wdeg_tables_vs_python_ac3(seed):
+    desc = ternary_model(seed, n=7, d=4, e=6, n_tab=3, ntup=20)
+    want, pos, neg, sat, weight = py_mac_wdeg(desc, node_limit=300)
+    o = oracle.Oracle(desc, oracle.AC3)
+    with oracle.var_heuristic(2):
+        r = o.mac(node_limit=300, trace_cap=1000)
+    assert [t[:4] for t in r.trace] == want
+    assert (r.num_positive, r.num_negative) == (pos, neg)
+    assert o.weights().tolist() == weight
+
+
+def triangle_model():
+    """0 != 1 (c0), 1 != 2 (c1), 0 != 2 (c2) over {0, 1}: unsatisfiable, and
+    arc consistent at the root."""
+    dom = [2, 2, 2]
+    scopes = [(0, 1), (1, 2), (0, 2)]
+    lists = [[(a, b) for a in range(2) for b in range(2) if a != b]] * 3
+    return from_tuples(dom, scopes, lists)
+
+
+def test_wdeg_triangle_closed_form():
+    # Root: nothing removed.  dom/wdeg keys 2/2 for every variable: 0=0.
+    # AC3 from [0]: c0 leaves D(1) = {1} (1 queued), c2 leaves D(2) = {1}
+    # (2 queued); popping 1, c0's stamp is newer than 1's, c1 revises 1
+    # against D(2) = {1} and empties it: c1's weight becomes 2.  Refuting
+    # 0=0 at level 0 runs the mirror image, D(1) = D(2) = {0}, and c1 empties
+    # D(1) again: weight 3.  No decision is left: unsatisfiable after one
+    # positive and one negative node.
+    o = oracle.Oracle(triangle_model())
+    with oracle.var_heuristic(2):
+        r = o.mac(trace_cap=16)
+    assert [t[:4] for t in r.trace] == [(1, 0, 0, 0), (0, 0, 0, 0)]
+    assert (r.num_sol, r.num_positive, r.num_negative) == (0, 1, 1)
+    assert o.weights().tolist() == [1.0, 3.0, 1.0]
+
+
+def test_wdeg_planted_fault_is_caught():
+    # fault 5: AC3 forgets to raise the weight
+    with oracle.fault(5):
+        o = oracle.Oracle(triangle_model())
+        with oracle.var_heuristic(2):
+            o.mac(trace_cap=16)
+        assert o.weights().tolist() != [1.0, 3.0, 1.0]
+        desc = workload.random_binary(16, 4, 50, 0.3, 901)
+        want, *_ = py_mac_wdeg(desc, node_limit=300)
+        with oracle.var_heuristic(2):
+            r = oracle.Oracle(desc).mac(node_limit=300, trace_cap=1000)
+        assert [t[:4] for t in r.trace] != want
+
+
+def test_mac_wdeg_on_a_reference_fixture():
+    # rand-23-23-253 (samples/bench), the first 60 nodes
+    desc, _ = load_golden("rand-23-23-253-131-46021_ext")
+    want, pos, neg, sat, weight = py_mac_wdeg(desc, node_limit=60)
+    o = oracle.Oracle(desc)
+    with oracle.var_heuristic(2):
+        r = o.mac(node_limit=60, trace_cap=200)
+    assert [t[:4] for t in r.trace] == want
+    assert (r.num_positive, r.num_negative) == (pos, neg)
+    assert o.weights().tolist() == weight
+
+
+def test_bf_mac_has_no_wdeg_form():
+    with oracle.var_heuristic(2):
+        with pytest.raises(ValueError):
+            oracle.bf_mac(triangle_model(), trace_cap=16)
