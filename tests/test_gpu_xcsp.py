@@ -53,3 +53,26 @@ def test_cli_sac_and_gac_only(gpu, capsys):
     assert "SAC: consistent" in out and "MAC stats:" in out
     code, _ = run_cli(capsys, os.path.join(XML, "missing.xml"))
     assert code == 2
+
+
+def test_tier_runner_over_the_fixtures(gpu, tmp_path, capsys):
+    # batch_test_v2.py counterpart: every file through the CLI in a child
+    # process, solutions verified against the files, results in CSV
+    import csv
+
+    from cpim_amd import tier
+    out = tmp_path / "tier.csv"
+    code = tier.main([XML, "--timeout", "3", "--verify", "--stats", "--export-csv", str(out)])
+    text = capsys.readouterr().out
+    assert code == 0, text
+    rows = {r["file"]: r for r in csv.DictReader(open(out))}
+    assert sorted(rows) == sorted(f for f in os.listdir(XML) if f.endswith(".xml"))
+    t = rows["test.xml"]
+    assert (t["status"], t["branches"], t["backtracks"], t["verified"]) == ("SAT", "3", "0", "OK")
+    assert (t["num_vars"], t["num_cons"]) == ("3", "2")
+    assert (rows["queens-4_ext.xml"]["status"], rows["queens-4_ext.xml"]["verified"]) == ("SAT", "OK")
+    assert rows["composed-25-1-2-2_ext.xml"]["status"] == "UNSAT"
+    for r in rows.values():
+        assert r["status"] in ("SAT", "UNSAT", "TIMEOUT"), r
+        assert r["verified"] == ("OK" if r["status"] == "SAT" else "N/A"), r
+    assert "Verified: OK=" in text

@@ -79,3 +79,44 @@ def test_errors(bad, msg):
 def test_expanded_conflicts_limit():
     with pytest.raises(xcsp.XcspError, match="exceeds limit"):
         xcsp.read(DOC, skip_unsupported=True, max_expanded_tuples=8)
+
+
+# ---------------------------------------------------------------------------
+# Tier runner (samples/batch_test_v2.py counterpart): host-side parts
+# ---------------------------------------------------------------------------
+def test_tier_expand_paths(tmp_path):
+    from cpim_amd import tier
+    lst = tmp_path / "tier0.txt"
+    lst.write_text("# smoke tier\n%s\n\n%s\n" % (os.path.join(XML, "test.xml"), "queens-4_ext.xml"))
+    files = tier.expand_paths([XML, "@" + str(lst)])
+    names = [os.path.basename(f) for f in files]
+    assert names == sorted(f for f in os.listdir(XML) if f.endswith(".xml")) + ["queens-4_ext.xml"]
+    # the list's relative entry resolves against the list's folder
+    assert files[-1] == os.path.join(str(tmp_path), "queens-4_ext.xml")
+
+
+def test_tier_parses_solver_output():
+    from cpim_amd import tier
+    out = ("Model q: 4 variables, 6 constraints\n"
+           "MAC solution (canonical indices): 1 3 0 2\n"
+           "MAC solution (original values): 2 4 1 3\n"
+           "MAC stats: time=12 ms, positives=5, negatives=1, nodes=6, timeout=false\n")
+    f = tier.parse_solver_output(out, 0)
+    assert (f["status"], f["time_ms"], f["branches"], f["backtracks"], f["num_vars"], f["num_cons"]) == \
+        ("SAT", 12, 5, 1, 4, 6)
+    assert f["solution"] == [1, 3, 0, 2]
+    f = tier.parse_solver_output("MAC search did not find a solution.\nMAC stats: time=3 ms, "
+                                 "positives=7, negatives=7, nodes=14, timeout=false\n", 1)
+    assert (f["status"], f["branches"], f["solution"]) == ("UNSAT", 7, None)
+    f = tier.parse_solver_output("MAC search timed out before finding a solution.\n", 1)
+    assert f["status"] == "TIMEOUT"
+    assert tier.parse_solver_output("error: no such file\n", 2)["status"].startswith("ERROR:no such")
+
+
+def test_tier_check_solution():
+    from cpim_amd import tier
+    inst = xcsp.read(os.path.join(XML, "queens-4_ext.xml"))
+    assert tier.check_solution(inst, [1, 3, 0, 2])
+    assert not tier.check_solution(inst, [0, 1, 2, 3])
+    assert not tier.check_solution(inst, [1, 3, 0])
+    assert not tier.check_solution(inst, [1, 3, 0, 9])
