@@ -21,7 +21,7 @@
 
 typedef unsigned long long u64;
 
-#define CPIM_MAX_ARITY 4
+#define CPIM_MAX_ARITY 8
 #define CPIM_MAX_WORDS 16   // domains up to 1024 values
 
 struct ConMeta {
@@ -231,18 +231,24 @@ __device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* s
 
 // ---------------------------------------------------------------------------
 // Table constraint revision (positive table, bitset per (pos, value) over the
-// tuple indices).  valid = AND over positions of OR over live values of
-// tsup[pos][value]; value b of the variable at position p keeps iff
-// tsup[p][b] AND valid is not empty.  Positions whose variable is `skip_var`,
-// or is assigned (asg bitmap, may be NULL), are not revised; with
-// only_var >= 0, only the position of that variable is.  *wiped is set if this wave
-// emptied a domain.  on_removed(var, count) is called (by the whole wave)
-// for every variable that lost values.  Requires tw <= 64 (4096 tuples).
+// tuple indices, tw words per row).  valid = AND over positions of OR over
+// live values of tsup[pos][value]; value b of the variable at position p
+// keeps iff tsup[p][b] AND valid is not empty.  Positions whose variable is
+// `skip_var`, or is assigned (asg bitmap, may be NULL), are not revised; with
+// only_var >= 0, only the position of that variable is.  *wiped is set if
+// this wave emptied a domain.  on_removed(var, count) is called (by the whole
+// wave) for every variable that lost values.  Any number of tuples and up to
+// CPIM_MAX_ARITY positions.
 //
-// The OR over live values runs with lanes = (value k, word w) pairs: one
-// round of independent row loads per 64 / Tp values (Tp = tw rounded up to a
-// power of two), then an OR across the lanes of each word.  The per-value
-// check loads the value's row four words at a time.
+// The tuple words are taken in chunks of 64 words (4096 tuples).  In a chunk,
+// the OR over live values runs with lanes = (value k, word w) pairs: one round
+// of independent row loads per 64 / Tp values (Tp = chunk words rounded up to
+// a power of two), then an OR across the lanes of each word.  The per-value
+// check loads the row four words at a time.  A table of one chunk computes
+// valid once for all positions.  A larger one is scanned per revised
+// position: a value supported in one chunk is not checked again, a chunk
+// with no valid tuple is skipped, and the scan stops once every value of the
+// position has a support.
 // ---------------------------------------------------------------------------
 // Index of the k-th set bit of the W-word domain held by lanes 0..W-1 of
 // d_l (k < popcount); lanes call it with their own k.
@@ -272,68 +278,104 @@ __device__ __forceinline__ void revise_table(const DevNet& net, u64* dom, int* s
   const int arity = mp->arity;
   const u64* ts = net.tsup + mp->off;
   const int tw = mp->tw;
-  const int Tp = pow2_at_least(tw);
-  const int per = 64 / Tp;  // values per round of loads
-  // valid tuples, word `lane` (lanes < tw)
-  u64 valid_l = ~0ull;
-  for (int p = 0; p < arity; ++p) {
-    const int v = mp->scope[p];
-    const u64 dv_l = lane < W ? ld_dom<kCoherent>(&dom[(long long)v * W + lane]) : 0ull;
-    int cnt = 0;
-    for (int j = 0; j < W; ++j) cnt += __popcll(readlane64(dv_l, j));
+  auto load_dom = [&](int v) -> u64 {
+    return lane < W ? ld_dom<kCoherent>(&dom[(long long)v * W + lane]) : 0ull;
+  };
+  auto revised = [&](int v) -> bool {
+    return v != skip_var && (only_var < 0 || v == only_var) && !(asg && ((asg[v >> 5] >> (v & 31)) & 1u));
+  };
+  // valid tuples of words [c0, c0 + cw), word `lane` (lanes < cw)
+  auto chunk_valid = [&](int c0, int cw) -> u64 {
+    const int Tp = pow2_at_least(cw);
+    const int per = 64 / Tp;  // values per round of loads
     const int w = lane & (Tp - 1);
-    u64 acc = 0ull;
-    for (int k0 = 0; k0 < cnt; k0 += per) {
-      const int k = k0 + lane / Tp;
-      if (w < tw && k < cnt) {
-        const int a = kth_value(dv_l, W, k);
-        acc |= ts[((long long)p * dpad + a) * tw + w];
+    u64 valid_l = ~0ull;
+    for (int q = 0; q < arity; ++q) {
+      const u64 dq_l = load_dom(mp->scope[q]);
+      int cnt = 0;
+      for (int j = 0; j < W; ++j) cnt += __popcll(readlane64(dq_l, j));
+      u64 acc = 0ull;
+      for (int k0 = 0; k0 < cnt; k0 += per) {
+        const int k = k0 + lane / Tp;
+        if (w < cw && k < cnt) {
+          const int a = kth_value(dq_l, W, k);
+          acc |= ts[((long long)q * dpad + a) * tw + c0 + w];
+        }
       }
+      for (int o = Tp; o < 64; o <<= 1) acc |= __shfl_xor(acc, o, 64);
+      valid_l &= acc;
     }
-    for (int o = Tp; o < 64; o <<= 1) acc |= __shfl_xor(acc, o, 64);
-    valid_l &= acc;
-  }
-  if (lane >= tw) valid_l = 0ull;
-  bool any_wipe = false;
-  for (int p = 0; p < arity; ++p) {
-    const int y = mp->scope[p];
-    if (y == skip_var) continue;
-    if (only_var >= 0 && y != only_var) continue;
-    if (asg && ((asg[y >> 5] >> (y & 31)) & 1u)) continue;
-    const u64 du_l = lane < W ? ld_dom<kCoherent>(&dom[(long long)y * W + lane]) : 0ull;
+    return lane < cw ? valid_l : 0ull;
+  };
+  // values of position p (domain du_l) not in done_l that have a valid tuple
+  // in words [c0, c0 + cw); lane j returns word j of the mask
+  auto supported = [&](int p, u64 du_l, u64 done_l, u64 valid_l, int c0, int cw) -> u64 {
     u64 mask_l = 0ull;
     for (int j = 0; j < W; ++j) {
-      const u64 duj = readlane64(du_l, j);
-      if (duj == 0ull) continue;
+      const u64 open = readlane64(du_l, j) & ~readlane64(done_l, j);
+      if (open == 0ull) continue;
       const int b = j * 64 + lane;
       bool keep = false;
-      if ((duj >> lane) & 1ull) {
-        const u64* row = ts + ((long long)p * dpad + b) * tw;
-        for (int w0 = 0; w0 < tw && !keep; w0 += 4) {
+      if ((open >> lane) & 1ull) {
+        const u64* row = ts + ((long long)p * dpad + b) * tw + c0;
+        for (int w0 = 0; w0 < cw && !keep; w0 += 4) {
           u64 r[4];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) r[q] = w0 + q < tw ? row[w0 + q] : 0ull;
+          for (int q = 0; q < 4; ++q) r[q] = w0 + q < cw ? row[w0 + q] : 0ull;
 #pragma unroll
           for (int q = 0; q < 4; ++q)
-            if (w0 + q < tw && (r[q] & readlane64(valid_l, w0 + q < tw ? w0 + q : 0))) keep = true;
+            if (w0 + q < cw && (r[q] & readlane64(valid_l, w0 + q < cw ? w0 + q : 0))) keep = true;
         }
       }
       const u64 mj = __ballot(keep);
       if (lane == j) mask_l = mj;
     }
+    return mask_l;
+  };
+  bool any_wipe = false;
+  auto apply = [&](int y, u64 du_l, u64 keep_l) {
     int removed_l = 0;
-    if (lane < W && (du_l & ~mask_l)) {
-      const u64 old = atomicAnd(&dom[(long long)y * W + lane], mask_l);
-      removed_l = __popcll(old & ~mask_l);
+    if (lane < W && (du_l & ~keep_l)) {
+      const u64 old = atomicAnd(&dom[(long long)y * W + lane], keep_l);
+      removed_l = __popcll(old & ~keep_l);
     }
     int removed = 0;
-    for (int w = 0; w < W; ++w) removed += __builtin_amdgcn_readlane(removed_l, w);
+    for (int j = 0; j < W; ++j) removed += __builtin_amdgcn_readlane(removed_l, j);
     if (removed > 0) {
       int old_size = 0;
       if (lane == 0) old_size = atomicSub(&size[y], removed);
       old_size = __builtin_amdgcn_readfirstlane(old_size);
       if (old_size == removed) any_wipe = true;
       on_removed(y, removed);
+    }
+  };
+  if (tw <= 64) {
+    // one chunk: valid once, then every revised position
+    const u64 valid_l = chunk_valid(0, tw);
+    for (int p = 0; p < arity; ++p) {
+      const int y = mp->scope[p];
+      if (!revised(y)) continue;
+      const u64 du_l = load_dom(y);
+      apply(y, du_l, supported(p, du_l, 0ull, valid_l, 0, tw));
+    }
+  } else {
+    // chunks of 4096 tuples per revised position, until every value of the
+    // position has a support.  A removal leaves the valid tuples unchanged
+    // (no valid tuple goes through a value without support), so the later
+    // positions may read the reduced domain.
+    for (int p = 0; p < arity; ++p) {
+      const int y = mp->scope[p];
+      if (!revised(y)) continue;
+      const u64 du_l = load_dom(y);
+      u64 keep_l = 0ull;
+      for (int c0 = 0; c0 < tw; c0 += 64) {
+        const int cw = min(64, tw - c0);
+        const u64 valid_l = chunk_valid(c0, cw);
+        if (__ballot(valid_l != 0ull) == 0ull) continue;
+        keep_l |= supported(p, du_l, keep_l, valid_l, c0, cw);
+        if (__ballot(lane < W && (du_l & ~keep_l) != 0ull) == 0ull) break;
+      }
+      apply(y, du_l, keep_l);
     }
   }
   *wiped = any_wipe;

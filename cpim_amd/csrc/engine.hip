@@ -187,11 +187,11 @@ ModelImage make_image(const cpim_model_desc* d, bool skip_non_binary, bool own_r
       ++bin_k;
     } else {
       if (skip_non_binary) continue;
-      REQUIRE(ar <= CPIM_MAX_ARITY, "tables of arity above 4 are not supported");
+      REQUIRE(ar <= CPIM_MAX_ARITY, "tables of arity above 8 are not supported");
       const long long t0 = d->tup_off[c], t1 = d->tup_off[c + 1];
       REQUIRE(t1 >= t0 && (t1 - t0) % ar == 0, "tup_off does not delimit whole tuples");
       const long long T = (t1 - t0) / ar;
-      REQUIRE(T <= 64 * 64, "tables above 4096 tuples are not supported");
+      REQUIRE(T < (1ll << 31), "tables of 2^31 tuples or more are not supported");
       for (int i = 0; i < ar; ++i)
         for (int j = i + 1; j < ar; ++j) REQUIRE(sc[i] != sc[j], "table scope repeats a variable");
       ConMeta m{};
@@ -201,6 +201,7 @@ ModelImage make_image(const cpim_model_desc* d, bool skip_non_binary, bool own_r
       m.tw = std::max<int>(1, (int)((T + 63) / 64));
       m.off = tsup_off;
       const long long words = (long long)ar * im.dpad * m.tw;
+      REQUIRE(tsup_off + words <= (1ll << 32), "table supports above 32 GiB are not supported");
       im.tsup.resize(tsup_off + words, 0ull);
       for (long long t = 0; t < T; ++t) {
         const int32_t* tup = d->tuples + t0 + t * ar;

@@ -37,3 +37,19 @@ def gpu():
     if code != 0:
         pytest.fail("no GPU visible: " + lib.cpim_last_error().decode())
     return info
+
+
+def large_table_model(seed, n=12, d=5, e=14, p=0.35,
+                      tabs=((6, 4500), (6, 4500), (7, 5000), (8, 5000))):
+    """Seeded random binary network plus positive tables of arity 6 to 8 with
+    4500 to 5000 tuples each (two to three 4096-tuple chunks per table)."""
+    from cpim_amd import workload
+    from cpim_amd.model import from_tuples
+    base = workload.random_binary(n, d, e, p, seed)
+    scopes = [list(s) for s in base.scopes()]
+    lists = list(base.tuple_lists())
+    rng = np.random.default_rng(seed)
+    for k, (ar, nt) in enumerate(tabs):
+        scopes.append(sorted(rng.choice(n, ar, replace=False).tolist()))
+        lists.append(workload.random_table(seed * 131 + k, ar, d, nt))
+    return from_tuples(base.dom_size, scopes, lists)

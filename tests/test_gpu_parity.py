@@ -474,6 +474,47 @@ def test_tables_gac_and_mac(gpu, seed):
         assert trace == r.trace, name
 
 
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_large_tables_gac_sac_mac(gpu, seed):
+    # tables of arity 6-8 with 4500-5000 tuples: the revision walks 4096-tuple
+    # chunks; every GAC mode, SAC, and the MAC under all three orderings
+    from conftest import large_table_model
+    desc = large_table_model(seed)
+    opts = GModelOptions(skip_non_binary=False)
+    ok, dom, _ = oracle.Oracle(desc, oracle.AC3).root_ac()
+    for mode, code in GAC_MODES.items():
+        with GModelAdapter.Build(desc, opts) as m:
+            st = m._gac(-1, code)
+            assert st.inconsistent == (not ok), mode
+            if ok:
+                assert (m.ReadDomains(0) == dom).all(), mode
+    sok, sdom, _ = oracle.Oracle(desc, oracle.AC3).sac1()
+    with GModelAdapter.Build(desc, opts) as m:
+        st = m.EnforceSAC()
+        assert st.consistent == sok
+        if sok:
+            assert (m.ReadDomains(0) == sdom).all()
+    for varh, name in ((0, "dom"), (1, "dom/deg"), (2, "dom/wdeg")):
+        with oracle.var_heuristic(varh):
+            r = oracle.Oracle(desc, oracle.AC3).mac(node_limit=2000, trace_cap=4000)
+        with GModelAdapter.Build(desc, opts) as m:
+            st, sol, trace = m.Solve(node_limit=2000, trace_capacity=4000, heuristic=name)
+        assert (st.num_solutions, st.num_positive, st.num_negative) == \
+            (r.num_sol, r.num_positive, r.num_negative), name
+        assert trace == r.trace, name
+        if r.num_sol:
+            assert list(sol) == list(r.solution), name
+
+
+def test_table_limits_reported(gpu):
+    # arity above 8 is refused with a clear error
+    from cpim_amd import workload
+    dom = [3] * 9
+    desc = from_tuples(dom, [list(range(9))], [workload.random_table(1, 9, 3, 50)])
+    with pytest.raises(N.CpimError, match="arity above 8"):
+        GModelAdapter.Build(desc, GModelOptions(skip_non_binary=False))
+
+
 # ---------------------------------------------------------------------------
 # Full benchmark sizes: properties + sampled oracle checks
 # ---------------------------------------------------------------------------

@@ -576,3 +576,28 @@ def test_bf_mac_has_no_wdeg_form():
     with oracle.var_heuristic(2):
         with pytest.raises(ValueError):
             oracle.bf_mac(triangle_model(), trace_cap=16)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_large_nary_tables_vs_bruteforce(seed):
+    # tables of arity 6-8 and more than 4096 tuples (generic AC3)
+    from conftest import large_table_model
+    desc = large_table_model(seed)
+    ok, dom, _ = oracle.Oracle(desc, oracle.AC3).root_ac()
+    bok, bdom = oracle.bf_gac(desc)
+    assert ok == bok and (not ok or (dom == bdom).all())
+    sok, sdom, _ = oracle.Oracle(desc, oracle.AC3).sac1()
+    bsok, bsdom = oracle.bf_sac(desc)
+    assert sok == bsok and (not sok or (sdom == bsdom).all())
+    r = oracle.Oracle(desc, oracle.AC3).mac(node_limit=300, trace_cap=1000)
+    b = oracle.bf_mac(desc, node_limit=300, trace_cap=1000)
+    assert (r.num_sol, r.num_positive, r.num_negative) == (b.num_sol, b.num_positive, b.num_negative)
+    assert r.trace == b.trace
+    assert r.num_negative > 0
+    if r.num_sol:
+        assert check_solution(desc, r.solution)
+    want, pos, neg, sat, weight = py_mac_wdeg(desc, node_limit=100)
+    o = oracle.Oracle(desc, oracle.AC3)
+    with oracle.var_heuristic(2):
+        rw = o.mac(node_limit=100, trace_cap=400)
+    assert [t[:4] for t in rw.trace] == want and o.weights().tolist() == weight
