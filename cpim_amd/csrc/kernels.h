@@ -1118,22 +1118,33 @@ __device__ __forceinline__ void clear_supported_wide(u64 (&pend)[M], unsigned bi
 
 // Nibble tables ("four Russians"): entry (q, s) of a half-word table holds,
 // for the 4 values 4q..4q+3 of the half, the OR of the masks of the values
-// in subset s.  One lookup covers every support of a row nibble.  An
-// exhausted half reads entry (7, 0) (v_ffbl_b32 gives -1, & 28 gives nibble
-// 7, and the subset of an empty word is 0), which is zero like every s = 0.
+// in subset s (s = 0 is zero).  One lookup covers every support of a row
+// nibble.  The nibbles are taken in order, kNibStep per step, by every lane
+// of the wave together: the lanes of a step read the same 16-entry blocks,
+// which keeps LDS bank conflicts low, and the loads of a step are
+// independent.  A lane stops once its row bits or its pending tests are
+// exhausted.  Four per step measured 8.2 ms per 128,000-test pass against
+// 8.8 ms for two and 8.4 ms for eight (and 10.0 ms for lanes that each jump
+// to their own first non-empty nibble, the earlier form).
+constexpr int kNibStep = 4;
+constexpr unsigned kNibStepMask = kNibStep >= 8 ? 0xffffffffu : ((1u << (4 * kNibStep)) - 1u);
+
 template <int M>
 __device__ __forceinline__ void clear_supported_nib(u64 (&pend)[M], unsigned bits, const u64* tab) {
-  while (bits && any_mask<M>(pend)) {
-    const unsigned p0 = (unsigned)ffbl_raw(bits) & 28u;
-    const unsigned s0 = (bits >> p0) & 15u;
-    bits &= ~(15u << p0);
-    const unsigned p1 = (unsigned)ffbl_raw(bits) & 28u;
-    const unsigned s1 = (bits >> p1) & 15u;
-    bits &= ~(15u << p1);
-    const u64* e0 = tab + (long long)((p0 << 2) | s0) * M;
-    const u64* e1 = tab + (long long)((p1 << 2) | s1) * M;
+  for (int q = 0; q < 8 && bits && any_mask<M>(pend); q += kNibStep) {
+    u64 acc[M];
 #pragma unroll
-    for (int m = 0; m < M; ++m) pend[m] &= ~(e0[m] | e1[m]);
+    for (int m = 0; m < M; ++m) acc[m] = 0ull;
+#pragma unroll
+    for (int i = 0; i < kNibStep; ++i) {
+      const unsigned s = (bits >> (4 * (q + i))) & 15u;
+      const u64* e = tab + (long long)(((q + i) << 4) | s) * M;
+#pragma unroll
+      for (int m = 0; m < M; ++m) acc[m] |= e[m];
+    }
+    bits &= ~(kNibStepMask << (4 * q));
+#pragma unroll
+    for (int m = 0; m < M; ++m) pend[m] &= ~acc[m];
   }
 }
 
