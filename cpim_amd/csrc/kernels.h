@@ -1056,15 +1056,17 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
 // sl[(v * dpad + b) * M + m] holding tests 64m..64m+63.  Within a round the
 // workgroup walks the changed variables t one at a time: all threads stage
 // t's masks into a shared, double-buffered LDS area (one barrier per t),
-// then the waves split t's arcs u <- t.  One support entry then serves 64 M
-// tests; the revision is the one of k_sliced_tests applied to M masks at
-// once (same fixpoint argument, same exact end-of-round wipe-out check).
+// then the waves take t's arcs u <- t from a shared counter.  One support
+// entry serves 64 M tests; the revision is the one of k_sliced_tests applied
+// to M masks at once (same fixpoint argument, same exact end-of-round
+// wipe-out check).
 // ---------------------------------------------------------------------------
 template <int M>
 struct WideCtl {
   int m;
   int nchanged[2];
   int pad;
+  int arc_ctr[2];  // next arc of the staged variable, per stage buffer
   u64 act[M];
   unsigned long long revs;
 };
@@ -1321,6 +1323,9 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
       u64* atw = atw0 + (i & 1) * W;
       // stage t: thread v holds value v's M masks; one wave covers one word
       const u64* slt = sl + (long long)t * dpad * M;
+      // every wave finished the arcs of buffer (i & 1)'s previous variable
+      // before the last barrier
+      if (tid == 0) ctl.arc_ctr[i & 1] = 0;
       if (kNib) {
         // nibble tables: item (q, m) ORs the 16 subsets of values 4q..4q+3
         for (int it = tid; it < 16 * W * M; it += nthreads) {
@@ -1353,7 +1358,12 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
       }
       __syncthreads();
       const int a0 = net.adj_off[t], deg = net.adj_off[t + 1] - a0;
-      for (int j = wave; j < deg; j += nwaves) {
+      // waves take t's arcs one at a time from a shared counter
+      while (true) {
+        int j = 0;
+        if (lane == 0) j = atomicAdd(&ctl.arc_ctr[i & 1], 1);
+        j = __builtin_amdgcn_readfirstlane(j);
+        if (j >= deg) break;
         const int4 e = net.adj[a0 + j];
         const int u = uniform_int(e.y);
         const int pos_t = uniform_int(e.z);
