@@ -92,3 +92,36 @@ def test_invalid_tests_rejected(gpu):
             m.SingletonTests(np.array([[10, 0]], np.int32))
         with pytest.raises(N.CpimInvalidArgument, match="outside the domain"):
             m.SingletonTests(np.array([[0, 8]], np.int32))
+
+
+def padded_pigeonhole(k=6, d=5, npad=3470):
+    """Pigeonhole core (k variables pairwise different over d < k values,
+    unsatisfiable, arc consistent) plus a chain of npad variables of size 8
+    under universal relations: every ordering decides the core first, and the
+    3,476 variables push the search state past the LDS budget."""
+    ne = np.array([(a, b) for a in range(d) for b in range(d) if a != b], np.int32)
+    univ = np.array([(a, b) for a in range(8) for b in range(8)], np.int32)
+    scopes, lists = [], []
+    for i in range(k):
+        for j in range(i + 1, k):
+            scopes.append((i, j))
+            lists.append(ne)
+    for v in range(k, k + npad - 1):
+        scopes.append((v, v + 1))
+        lists.append(univ)
+    return from_tuples([d] * k + [8] * npad, scopes, lists)
+
+
+@pytest.mark.parametrize("heuristic,varh", [("dom", 0), ("dom/deg", 1), ("dom/wdeg", 2)])
+def test_mac_in_global_scratch(gpu, heuristic, varh):
+    # the device MAC keeps domains, queues and stamps in global scratch here;
+    # about 140 failures each, so dom/wdeg's weights change all along
+    desc = padded_pigeonhole()
+    with oracle.var_heuristic(varh):
+        r = oracle.Oracle(desc).mac(node_limit=400, trace_cap=800)
+    assert r.num_negative > 100 and not r.num_sol
+    with GModelAdapter.Build(desc) as m:
+        st, sol, trace = m.Solve(node_limit=400, trace_capacity=800, heuristic=heuristic)
+    assert (st.num_solutions, st.num_positive, st.num_negative) == \
+        (r.num_sol, r.num_positive, r.num_negative)
+    assert trace == r.trace

@@ -76,3 +76,15 @@ def test_tier_runner_over_the_fixtures(gpu, tmp_path, capsys):
         assert r["status"] in ("SAT", "UNSAT", "TIMEOUT"), r
         assert r["verified"] == ("OK" if r["status"] == "SAT" else "N/A"), r
     assert "Verified: OK=" in text
+
+
+@pytest.mark.parametrize("varh,code", [("dom/deg", 1), ("dom/wdeg", 2)])
+def test_cli_variable_orderings(gpu, capsys, varh, code):
+    path = os.path.join(XML, "queens-12_ext.xml")
+    inst = xcsp.read(path)
+    with oracle.var_heuristic(code):
+        r = oracle.Oracle(inst.desc).mac(node_limit=20000)
+    rc, out = run_cli(capsys, path, "--varh", varh, "--node-limit", "20000")
+    m = re.search(r"positives=(\d+), negatives=(\d+)", out)
+    assert m and (int(m.group(1)), int(m.group(2))) == (r.num_positive, r.num_negative)
+    assert (rc == 0) == bool(r.num_sol)
