@@ -218,6 +218,138 @@ __device__ __forceinline__ int revise_binary_at(const DevNet& net, u64* dom, int
   return removed;
 }
 
+// Index of the k-th set bit of the W-word domain held by lanes 0..W-1 of
+// d_l (k < popcount); lanes call it with their own k.
+__device__ __forceinline__ int kth_value(u64 d_l, int W, int k) {
+  int a = -1;
+  for (int j = 0; j < W && a < 0; ++j) {
+    u64 x = readlane64(d_l, j);
+    const int c = __popcll(x);
+    if (k < c) {
+      for (int i = 0; i < k; ++i) x &= x - 1;
+      a = j * 64 + (int)__builtin_ctzll(x);
+    } else {
+      k -= c;
+    }
+  }
+  return a;
+}
+
+// Two arcs uA <- t and uB <- t with the same source t, revised together by
+// one wave: the same result as two revise_binary_at calls in a row, with the
+// row loads of both arcs in flight at once (a wave's revisions are bound by
+// the latency of those loads).  uA may equal uB (parallel constraints): both
+// masks are then computed from the same D(u) and applied one after the
+// other, which removes exactly the values either revision removes.
+__device__ __forceinline__ void revise_binary_pair(const DevNet& net, u64* dom, int* size, int t,
+                                                   long long offA, int uA, int pos_tA,
+                                                   long long offB, int uB, int pos_tB,
+                                                   int* removedA, bool* wipedA, int* removedB,
+                                                   bool* wipedB) {
+  const int W = net.W;
+  const int dpad = net.dpad;
+  const int lane = lane_id();
+  const u64* baseA = net.sup + offA;
+  const u64* baseB = net.sup + offB;
+  const u64* plane_uA = baseA + (long long)(1 - pos_tA) * W * dpad;
+  const u64* plane_tA = baseA + (long long)pos_tA * W * dpad;
+  const u64* plane_uB = baseB + (long long)(1 - pos_tB) * W * dpad;
+  const u64* plane_tB = baseB + (long long)pos_tB * W * dpad;
+  const u64 dt_l = lane < W ? ld_dom<false>(&dom[(long long)t * W + lane]) : 0ull;
+  const u64 duA_l = lane < W ? ld_dom<false>(&dom[(long long)uA * W + lane]) : 0ull;
+  const u64 duB_l = lane < W ? ld_dom<false>(&dom[(long long)uB * W + lane]) : 0ull;
+  int cnt_t = 0;
+  int first_w = -1;
+  for (int w = 0; w < W; ++w) {
+    const u64 x = readlane64(dt_l, w);
+    cnt_t += __popcll(x);
+    if (first_w < 0 && x) first_w = w;
+  }
+  const int Wp = pow2_at_least(W);
+  u64 maskA_l = 0ull, maskB_l = 0ull;
+  if (cnt_t * Wp <= 64) {
+    // source side: lane (k, w) loads word w of the rows of t's k-th value
+    const int w = lane & (Wp - 1);
+    const int k = lane / Wp;
+    u64 rowA = 0ull, rowB = 0ull;
+    if (w < W && k < cnt_t) {
+      const int a = kth_value(dt_l, W, k);
+      rowA = plane_tA[(long long)w * dpad + a];
+      rowB = plane_tB[(long long)w * dpad + a];
+    }
+    for (int o = Wp; o < 64; o <<= 1) {
+      rowA |= __shfl_xor(rowA, o, 64);
+      rowB |= __shfl_xor(rowB, o, 64);
+    }
+    maskA_l = lane < W ? rowA : 0ull;
+    maskB_l = lane < W ? rowB : 0ull;
+  } else {
+    // target side: lanes are 64 values of u, the word of D(t)'s first
+    // non-empty word first, for both arcs and every live chunk at once
+    const u64 dt0 = readlane64(dt_l, first_w);
+    const u64* plane_uA0 = plane_uA + (long long)first_w * dpad;
+    const u64* plane_uB0 = plane_uB + (long long)first_w * dpad;
+    for (int j0 = 0; j0 < W; j0 += 4) {
+      u64 rowA[4], rowB[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = j0 + q;
+        const bool liveA = j < W && readlane64(duA_l, j < W ? j : 0) != 0ull;
+        const bool liveB = j < W && readlane64(duB_l, j < W ? j : 0) != 0ull;
+        rowA[q] = liveA ? plane_uA0[j * 64 + lane] : 0ull;
+        rowB[q] = liveB ? plane_uB0[j * 64 + lane] : 0ull;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = j0 + q;
+        if (j >= W) break;
+        const int b = j * 64 + lane;
+        const u64 dA = readlane64(duA_l, j), dB = readlane64(duB_l, j);
+        bool keepA = false, keepB = false;
+        if ((dA >> lane) & 1ull) {
+          keepA = (rowA[q] & dt0) != 0ull;
+          for (int w = first_w + 1; w < W && !keepA; ++w) {
+            const u64 dtw = readlane64(dt_l, w);
+            if (dtw && (plane_uA[(long long)w * dpad + b] & dtw)) keepA = true;
+          }
+        }
+        if ((dB >> lane) & 1ull) {
+          keepB = (rowB[q] & dt0) != 0ull;
+          for (int w = first_w + 1; w < W && !keepB; ++w) {
+            const u64 dtw = readlane64(dt_l, w);
+            if (dtw && (plane_uB[(long long)w * dpad + b] & dtw)) keepB = true;
+          }
+        }
+        const u64 mA = __ballot(keepA), mB = __ballot(keepB);
+        if (lane == j) {
+          maskA_l = mA;
+          maskB_l = mB;
+        }
+      }
+    }
+  }
+  auto apply = [&](int u, u64 du_l, u64 mask_l, int* removed_out, bool* wiped_out) {
+    int removed_l = 0;
+    if (lane < W && (du_l & ~mask_l)) {
+      const u64 old = atomicAnd(&dom[(long long)u * W + lane], mask_l);
+      removed_l = __popcll(old & ~mask_l);
+    }
+    int removed = 0;
+    for (int w = 0; w < W; ++w) removed += __builtin_amdgcn_readlane(removed_l, w);
+    bool wipe = false;
+    if (removed > 0) {
+      int old_size = 0;
+      if (lane == 0) old_size = atomicSub(&size[u], removed);
+      old_size = __builtin_amdgcn_readfirstlane(old_size);
+      wipe = old_size == removed;
+    }
+    *removed_out = removed;
+    *wiped_out = wipe;
+  };
+  apply(uA, duA_l, maskA_l, removedA, wipedA);
+  apply(uB, duB_l, maskB_l, removedB, wipedB);
+}
+
 // Offset in sup of the planes of a binary arc's constraint.
 __device__ __forceinline__ long long arc_plane_off(const DevNet& net, const int4& e) {
   return (long long)e.w * 2 * net.W * net.dpad;
@@ -250,22 +382,6 @@ __device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* s
 // with no valid tuple is skipped, and the scan stops once every value of the
 // position has a support.
 // ---------------------------------------------------------------------------
-// Index of the k-th set bit of the W-word domain held by lanes 0..W-1 of
-// d_l (k < popcount); lanes call it with their own k.
-__device__ __forceinline__ int kth_value(u64 d_l, int W, int k) {
-  int a = -1;
-  for (int j = 0; j < W && a < 0; ++j) {
-    u64 x = readlane64(d_l, j);
-    const int c = __popcll(x);
-    if (k < c) {
-      for (int i = 0; i < k; ++i) x &= x - 1;
-      a = j * 64 + (int)__builtin_ctzll(x);
-    } else {
-      k -= c;
-    }
-  }
-  return a;
-}
 
 template <typename OnRemoved, bool kCoherent = false>
 __device__ __forceinline__ void revise_table(const DevNet& net, u64* dom, int* size, int c,

@@ -462,7 +462,9 @@ __device__ __forceinline__ void wg_mark(unsigned* nxt, WgCtl* ctl, int v, int sl
 // as GModel::EnforceGAC(assigned_var) does).  Returns false on wipe-out.
 // Must be called by every thread of the workgroup.  Work items are runs of
 // up to kWgArcs arcs of one changed variable: the lanes of a wave fetch the
-// arcs' metadata together, then the wave revises them one by one.
+// arcs' metadata together, then the wave revises them two at a time
+// (revise_binary_pair keeps the row loads of both in flight), or one by one
+// for tables and assigned targets.
 constexpr int kWgArcs = 8;  // most arcs per work item
 
 __device__ __forceinline__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_dirs_round0) {
@@ -544,6 +546,24 @@ __device__ __forceinline__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl*
         const int c = __builtin_amdgcn_readlane(e.x, i);
         const int u = __builtin_amdgcn_readlane(e.y, i);
         bool wiped = false;
+        // two binary arcs with live targets: revised together
+        if (u >= 0 && !both && i + 1 < cnt) {
+          const int u2 = __builtin_amdgcn_readlane(e.y, i + 1);
+          if (u2 >= 0 && !uniform_int((int)bit_of(mem.asg, u)) && !uniform_int((int)bit_of(mem.asg, u2))) {
+            const long long oA = (long long)__builtin_amdgcn_readlane(e.w, i) * 2 * net.W * net.dpad;
+            const long long oB = (long long)__builtin_amdgcn_readlane(e.w, i + 1) * 2 * net.W * net.dpad;
+            int rA = 0, rB = 0;
+            bool wA = false, wB = false;
+            revise_binary_pair(net, mem.dom, mem.size, t, oA, u, __builtin_amdgcn_readlane(e.z, i), oB, u2,
+                               __builtin_amdgcn_readlane(e.z, i + 1), &rA, &wA, &rB, &wB);
+            revs += 2;
+            if (rA && lane == 0) wg_mark(nxt, ctl, u, slot);
+            if (rB && lane == 0) wg_mark(nxt, ctl, u2, slot);
+            if ((wA || wB) && lane == 0) ctl->wipe = 1;
+            ++i;
+            continue;
+          }
+        }
         if (u >= 0) {  // binary
           const int pos_t = __builtin_amdgcn_readlane(e.z, i);
           const long long o = (long long)__builtin_amdgcn_readlane(e.w, i) * 2 * net.W * net.dpad;
