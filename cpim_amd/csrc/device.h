@@ -235,29 +235,24 @@ __device__ __forceinline__ int kth_value(u64 d_l, int W, int k) {
   return a;
 }
 
-// Two arcs uA <- t and uB <- t with the same source t, revised together by
-// one wave: the same result as two revise_binary_at calls in a row, with the
-// row loads of both arcs in flight at once (a wave's revisions are bound by
-// the latency of those loads).  uA may equal uB (parallel constraints): both
+// P arcs u[p] <- t with the same source t, revised together by one wave:
+// the same result as P revise_binary_at calls in a row, with the row loads
+// of all P arcs in flight at once (a wave's revisions are bound by the
+// latency of those loads).  Targets may repeat (parallel constraints): the
 // masks are then computed from the same D(u) and applied one after the
 // other, which removes exactly the values either revision removes.
-__device__ __forceinline__ void revise_binary_pair(const DevNet& net, u64* dom, int* size, int t,
-                                                   long long offA, int uA, int pos_tA,
-                                                   long long offB, int uB, int pos_tB,
-                                                   int* removedA, bool* wipedA, int* removedB,
-                                                   bool* wipedB) {
+template <int P>
+__device__ __forceinline__ void revise_binary_multi(const DevNet& net, u64* dom, int* size, int t,
+                                                    const long long (&off)[P], const int (&u)[P],
+                                                    const int (&pos_t)[P], int (&removed)[P],
+                                                    bool (&wiped)[P]) {
   const int W = net.W;
   const int dpad = net.dpad;
   const int lane = lane_id();
-  const u64* baseA = net.sup + offA;
-  const u64* baseB = net.sup + offB;
-  const u64* plane_uA = baseA + (long long)(1 - pos_tA) * W * dpad;
-  const u64* plane_tA = baseA + (long long)pos_tA * W * dpad;
-  const u64* plane_uB = baseB + (long long)(1 - pos_tB) * W * dpad;
-  const u64* plane_tB = baseB + (long long)pos_tB * W * dpad;
   const u64 dt_l = lane < W ? ld_dom<false>(&dom[(long long)t * W + lane]) : 0ull;
-  const u64 duA_l = lane < W ? ld_dom<false>(&dom[(long long)uA * W + lane]) : 0ull;
-  const u64 duB_l = lane < W ? ld_dom<false>(&dom[(long long)uB * W + lane]) : 0ull;
+  u64 du_l[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) du_l[p] = lane < W ? ld_dom<false>(&dom[(long long)u[p] * W + lane]) : 0ull;
   int cnt_t = 0;
   int first_w = -1;
   for (int w = 0; w < W; ++w) {
@@ -266,88 +261,87 @@ __device__ __forceinline__ void revise_binary_pair(const DevNet& net, u64* dom, 
     if (first_w < 0 && x) first_w = w;
   }
   const int Wp = pow2_at_least(W);
-  u64 maskA_l = 0ull, maskB_l = 0ull;
+  u64 mask_l[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) mask_l[p] = 0ull;
   if (cnt_t * Wp <= 64) {
     // source side: lane (k, w) loads word w of the rows of t's k-th value
     const int w = lane & (Wp - 1);
     const int k = lane / Wp;
-    u64 rowA = 0ull, rowB = 0ull;
+    u64 row[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) row[p] = 0ull;
     if (w < W && k < cnt_t) {
       const int a = kth_value(dt_l, W, k);
-      rowA = plane_tA[(long long)w * dpad + a];
-      rowB = plane_tB[(long long)w * dpad + a];
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+        row[p] = net.sup[off[p] + (long long)pos_t[p] * W * dpad + (long long)w * dpad + a];
     }
     for (int o = Wp; o < 64; o <<= 1) {
-      rowA |= __shfl_xor(rowA, o, 64);
-      rowB |= __shfl_xor(rowB, o, 64);
-    }
-    maskA_l = lane < W ? rowA : 0ull;
-    maskB_l = lane < W ? rowB : 0ull;
-  } else {
-    // target side: lanes are 64 values of u, the word of D(t)'s first
-    // non-empty word first, for both arcs and every live chunk at once
-    const u64 dt0 = readlane64(dt_l, first_w);
-    const u64* plane_uA0 = plane_uA + (long long)first_w * dpad;
-    const u64* plane_uB0 = plane_uB + (long long)first_w * dpad;
-    for (int j0 = 0; j0 < W; j0 += 4) {
-      u64 rowA[4], rowB[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int j = j0 + q;
-        const bool liveA = j < W && readlane64(duA_l, j < W ? j : 0) != 0ull;
-        const bool liveB = j < W && readlane64(duB_l, j < W ? j : 0) != 0ull;
-        rowA[q] = liveA ? plane_uA0[j * 64 + lane] : 0ull;
-        rowB[q] = liveB ? plane_uB0[j * 64 + lane] : 0ull;
+      for (int p = 0; p < P; ++p) row[p] |= __shfl_xor(row[p], o, 64);
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) mask_l[p] = lane < W ? row[p] : 0ull;
+  } else {
+    // target side: lanes are 64 values of u; the word of D(t)'s first
+    // non-empty word is loaded for every arc and live chunk before any test
+    const u64 dt0 = readlane64(dt_l, first_w);
+    for (int j0 = 0; j0 < W; j0 += 4) {
+      u64 row[P][4];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const u64* plane_u0 = net.sup + off[p] + (long long)(1 - pos_t[p]) * W * dpad + (long long)first_w * dpad;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = j0 + q;
+          const bool live = j < W && readlane64(du_l[p], j < W ? j : 0) != 0ull;
+          row[p][q] = live ? plane_u0[j * 64 + lane] : 0ull;
+        }
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int j = j0 + q;
         if (j >= W) break;
         const int b = j * 64 + lane;
-        const u64 dA = readlane64(duA_l, j), dB = readlane64(duB_l, j);
-        bool keepA = false, keepB = false;
-        if ((dA >> lane) & 1ull) {
-          keepA = (rowA[q] & dt0) != 0ull;
-          for (int w = first_w + 1; w < W && !keepA; ++w) {
-            const u64 dtw = readlane64(dt_l, w);
-            if (dtw && (plane_uA[(long long)w * dpad + b] & dtw)) keepA = true;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const u64 d = readlane64(du_l[p], j);
+          if (d == 0ull) continue;
+          const u64* plane_u = net.sup + off[p] + (long long)(1 - pos_t[p]) * W * dpad;
+          bool keep = false;
+          if ((d >> lane) & 1ull) {
+            keep = (row[p][q] & dt0) != 0ull;
+            for (int w = first_w + 1; w < W && !keep; ++w) {
+              const u64 dtw = readlane64(dt_l, w);
+              if (dtw && (plane_u[(long long)w * dpad + b] & dtw)) keep = true;
+            }
           }
-        }
-        if ((dB >> lane) & 1ull) {
-          keepB = (rowB[q] & dt0) != 0ull;
-          for (int w = first_w + 1; w < W && !keepB; ++w) {
-            const u64 dtw = readlane64(dt_l, w);
-            if (dtw && (plane_uB[(long long)w * dpad + b] & dtw)) keepB = true;
-          }
-        }
-        const u64 mA = __ballot(keepA), mB = __ballot(keepB);
-        if (lane == j) {
-          maskA_l = mA;
-          maskB_l = mB;
+          const u64 m = __ballot(keep);
+          if (lane == j) mask_l[p] = m;
         }
       }
     }
   }
-  auto apply = [&](int u, u64 du_l, u64 mask_l, int* removed_out, bool* wiped_out) {
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
     int removed_l = 0;
-    if (lane < W && (du_l & ~mask_l)) {
-      const u64 old = atomicAnd(&dom[(long long)u * W + lane], mask_l);
-      removed_l = __popcll(old & ~mask_l);
+    if (lane < W && (du_l[p] & ~mask_l[p])) {
+      const u64 old = atomicAnd(&dom[(long long)u[p] * W + lane], mask_l[p]);
+      removed_l = __popcll(old & ~mask_l[p]);
     }
-    int removed = 0;
-    for (int w = 0; w < W; ++w) removed += __builtin_amdgcn_readlane(removed_l, w);
+    int r = 0;
+    for (int w = 0; w < W; ++w) r += __builtin_amdgcn_readlane(removed_l, w);
     bool wipe = false;
-    if (removed > 0) {
+    if (r > 0) {
       int old_size = 0;
-      if (lane == 0) old_size = atomicSub(&size[u], removed);
+      if (lane == 0) old_size = atomicSub(&size[u[p]], r);
       old_size = __builtin_amdgcn_readfirstlane(old_size);
-      wipe = old_size == removed;
+      wipe = old_size == r;
     }
-    *removed_out = removed;
-    *wiped_out = wipe;
-  };
-  apply(uA, duA_l, maskA_l, removedA, wipedA);
-  apply(uB, duB_l, maskB_l, removedB, wipedB);
+    removed[p] = r;
+    wiped[p] = wipe;
+  }
 }
 
 // Offset in sup of the planes of a binary arc's constraint.

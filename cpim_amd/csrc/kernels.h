@@ -462,11 +462,13 @@ __device__ __forceinline__ void wg_mark(unsigned* nxt, WgCtl* ctl, int v, int sl
 // as GModel::EnforceGAC(assigned_var) does).  Returns false on wipe-out.
 // Must be called by every thread of the workgroup.  Work items are runs of
 // up to kWgArcs arcs of one changed variable: the lanes of a wave fetch the
-// arcs' metadata together, then the wave revises them two at a time
-// (revise_binary_pair keeps the row loads of both in flight), or one by one
-// for tables and assigned targets.
+// arcs' metadata together, then the wave revises kP binary arcs at a time
+// (revise_binary_multi keeps the row loads of all in flight), or one by one
+// for tables and assigned targets.  kP = 4 for the MAC kernels (uneven batch
+// 47 ms against 50 ms at kP = 2); 2 elsewhere, where kP = 4 costs occupancy.
 constexpr int kWgArcs = 8;  // most arcs per work item
 
+template <int kP = 2>
 __device__ __forceinline__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl* ctl, bool both_dirs_round0) {
   const int n = net.n;
   const int nvw = (n + 31) / 32;
@@ -546,20 +548,51 @@ __device__ __forceinline__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl*
         const int c = __builtin_amdgcn_readlane(e.x, i);
         const int u = __builtin_amdgcn_readlane(e.y, i);
         bool wiped = false;
-        // two binary arcs with live targets: revised together
-        if (u >= 0 && !both && i + 1 < cnt) {
-          const int u2 = __builtin_amdgcn_readlane(e.y, i + 1);
-          if (u2 >= 0 && !uniform_int((int)bit_of(mem.asg, u)) && !uniform_int((int)bit_of(mem.asg, u2))) {
-            const long long oA = (long long)__builtin_amdgcn_readlane(e.w, i) * 2 * net.W * net.dpad;
-            const long long oB = (long long)__builtin_amdgcn_readlane(e.w, i + 1) * 2 * net.W * net.dpad;
-            int rA = 0, rB = 0;
-            bool wA = false, wB = false;
-            revise_binary_pair(net, mem.dom, mem.size, t, oA, u, __builtin_amdgcn_readlane(e.z, i), oB, u2,
-                               __builtin_amdgcn_readlane(e.z, i + 1), &rA, &wA, &rB, &wB);
+        // up to kP binary arcs with live targets: revised together
+        if (u >= 0 && !both) {
+          int np = 1;
+          while (np < kP && i + np < cnt) {
+            const int un = __builtin_amdgcn_readlane(e.y, i + np);
+            if (un < 0 || uniform_int((int)bit_of(mem.asg, un))) break;
+            ++np;
+          }
+          if (np >= 2 && !uniform_int((int)bit_of(mem.asg, u))) {
+            if (kP >= 4 && np >= 4) {
+              long long o[4];
+              int uu[4], pt[4], r[4];
+              bool w[4];
+#pragma unroll
+              for (int p = 0; p < 4; ++p) {
+                o[p] = (long long)__builtin_amdgcn_readlane(e.w, i + p) * 2 * net.W * net.dpad;
+                uu[p] = __builtin_amdgcn_readlane(e.y, i + p);
+                pt[p] = __builtin_amdgcn_readlane(e.z, i + p);
+              }
+              revise_binary_multi<4>(net, mem.dom, mem.size, t, o, uu, pt, r, w);
+              revs += 4;
+#pragma unroll
+              for (int p = 0; p < 4; ++p) {
+                if (r[p] && lane == 0) wg_mark(nxt, ctl, uu[p], slot);
+                if (w[p] && lane == 0) ctl->wipe = 1;
+              }
+              i += 3;
+              continue;
+            }
+            long long o[2];
+            int uu[2], pt[2], r[2];
+            bool w[2];
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+              o[p] = (long long)__builtin_amdgcn_readlane(e.w, i + p) * 2 * net.W * net.dpad;
+              uu[p] = __builtin_amdgcn_readlane(e.y, i + p);
+              pt[p] = __builtin_amdgcn_readlane(e.z, i + p);
+            }
+            revise_binary_multi<2>(net, mem.dom, mem.size, t, o, uu, pt, r, w);
             revs += 2;
-            if (rA && lane == 0) wg_mark(nxt, ctl, u, slot);
-            if (rB && lane == 0) wg_mark(nxt, ctl, u2, slot);
-            if ((wA || wB) && lane == 0) ctl->wipe = 1;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+              if (r[p] && lane == 0) wg_mark(nxt, ctl, uu[p], slot);
+              if (w[p] && lane == 0) ctl->wipe = 1;
+            }
             ++i;
             continue;
           }
@@ -1707,7 +1740,7 @@ __device__ __forceinline__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* 
     wg_recount(mem, n, W);
     wg_fill_bitmap(mem.cur, n);
     __syncthreads();
-    const bool ok = A.weight ? wg_ac3(net, mem, ctl, A, -1) : wg_fixpoint(net, mem, ctl, false);
+    const bool ok = A.weight ? wg_ac3(net, mem, ctl, A, -1) : wg_fixpoint<4>(net, mem, ctl, false);
     if (ok) wg_copy_words(A.trail, mem.dom, nw);
     if (A.root_out) wg_copy_words(A.root_out, mem.dom, nw);
     status = ok ? MAC_RUNNING : MAC_UNSAT;
@@ -1775,7 +1808,7 @@ __device__ __forceinline__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* 
     ++npos;
     ++nodes_here;
     __syncthreads();
-    bool ok = A.weight ? wg_ac3(net, mem, ctl, A, x) : wg_fixpoint(net, mem, ctl, false);
+    bool ok = A.weight ? wg_ac3(net, mem, ctl, A, x) : wg_fixpoint<4>(net, mem, ctl, false);
     wg_trace(net, mem, ctl, A, 1, x, a, ok);
     if (ok && depth == n) {
       if (A.solution)
@@ -1813,7 +1846,7 @@ __device__ __forceinline__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* 
       } else {
         if (threadIdx.x == 0) mem.cur[dx >> 5] |= 1u << (dx & 31);
         __syncthreads();
-        ok = A.weight ? wg_ac3(net, mem, ctl, A, dx) : wg_fixpoint(net, mem, ctl, false);
+        ok = A.weight ? wg_ac3(net, mem, ctl, A, dx) : wg_fixpoint<4>(net, mem, ctl, false);
       }
       wg_trace(net, mem, ctl, A, 0, dx, dv, ok);
     }
