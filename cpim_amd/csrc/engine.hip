@@ -1215,6 +1215,65 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
   }
   const long long groups = (long long)gfirst.size();
   gfirst.push_back((int)count);
+  // Place the groups on workgroups by expected work.  The pass runs as one
+  // wave of workgroups, two per CU; workgroups bid and bid + cus share a CU
+  // (dispatch fills one slot per CU, then the second).  The heaviest groups
+  // (by the summed degree of their variables) get the CUs that hold a single
+  // workgroup, and every other heavy group shares its CU with a light one, so
+  // that the kernel does not wait on two heavy groups sharing a CU.  With more
+  // groups than slots, heavy groups go first.  (C3: 8.1 -> 7.1 ms per pass.)
+  if (wide && groups > 1 && groups <= kSliceBudgetBytes / (8ll * m->dm.n * m->dm.dpad * M)) {
+    const int cus = std::max(1, m->props.cus);
+    std::vector<long long> w((size_t)groups, 0);
+    for (long long g = 0; g < groups; ++g) {
+      int last = -1;
+      for (int i = gfirst[(size_t)g]; i < gfirst[(size_t)g + 1]; ++i) {
+        const int x = sorted[i].x;
+        if (x != last) w[(size_t)g] += m->im.adj_off[x + 1] - m->im.adj_off[x];
+        last = x;
+      }
+    }
+    std::vector<int> rank((size_t)groups);
+    for (long long g = 0; g < groups; ++g) rank[(size_t)g] = (int)g;
+    std::stable_sort(rank.begin(), rank.end(), [&](int a, int b) { return w[(size_t)a] > w[(size_t)b]; });
+    const int B = (int)groups;
+    std::vector<int> at_bid(rank);  // more than two per CU, or one: heaviest first
+    if (B > cus && B <= 2 * cus) {
+      int next = 0;
+      for (int k = B - cus; k < cus; ++k) at_bid[(size_t)k] = rank[(size_t)next++];
+      for (int k = 0; k < B - cus; ++k) {
+        at_bid[(size_t)k] = rank[(size_t)(next + k)];
+        at_bid[(size_t)(k + cus)] = rank[(size_t)(B - 1 - k)];
+      }
+    }
+    // the kernel takes group gi = xcd * q + min(xcd, rem) + (bid >> 3) for bid
+    const int q = B >> 3, rem = B & 7;
+    std::vector<int> gorder((size_t)B);
+    for (int bid = 0; bid < B; ++bid) {
+      const int xcd = bid & 7;
+      gorder[(size_t)(xcd * q + (xcd < rem ? xcd : rem) + (bid >> 3))] = at_bid[(size_t)bid];
+    }
+    // rebuild the test list in that group order
+    std::vector<int2> re((size_t)count);
+    std::vector<int> reorder((size_t)count), nf;
+    nf.reserve((size_t)B + 1);
+    long long pos = 0;
+    for (int gi = 0; gi < B; ++gi) {
+      const int g = gorder[(size_t)gi];
+      nf.push_back((int)pos);
+      for (int i = gfirst[(size_t)g]; i < gfirst[(size_t)g + 1]; ++i) {
+        re[(size_t)pos] = sorted[i];
+        reorder[(size_t)pos] = sorted_in ? i : order[(size_t)i];
+        ++pos;
+      }
+    }
+    nf.push_back((int)count);
+    sorted_buf.swap(re);
+    sorted = sorted_buf.data();
+    order.swap(reorder);
+    gfirst.swap(nf);
+    sorted_in = false;
+  }
   if (count > S.scap) {
     S.stests = DevAlloc(sizeof(int2) * count);
     S.perm = DevAlloc(sizeof(int) * count);
