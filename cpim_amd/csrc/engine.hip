@@ -1176,7 +1176,8 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
                           cpim_sac_stats* st, SacScratch& S) {
   if (count == 0) return;
   REQUIRE(count <= INT_MAX, "too many singleton tests for one call");
-  // Lists sorted by variable (every SAC pass) are used as they are.
+  // Tests are grouped by variable: a list sorted by variable (every SAC pass)
+  // needs no sort.
   bool sorted_in = true;
   for (long long i = 1; i < count && sorted_in; ++i) sorted_in = tests_host[i - 1].x <= tests_host[i].x;
   std::vector<int> order;
