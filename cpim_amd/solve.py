@@ -38,6 +38,9 @@ def main(argv=None) -> int:
     ap.add_argument("--skip-unsupported", action="store_true",
                     help="ignore constraints that reference no extension relation")
     a = ap.parse_args(argv)
+    if a.dway and a.varh != "dom":
+        print("error: the d-way search orders variables by smallest domain only", file=sys.stderr)
+        return 2
     try:
         inst = xcsp.read(a.file, skip_unsupported=a.skip_unsupported)
     except (xcsp.XcspError, OSError) as e:

@@ -120,3 +120,9 @@ def test_tier_check_solution():
     assert not tier.check_solution(inst, [0, 1, 2, 3])
     assert not tier.check_solution(inst, [1, 3, 0])
     assert not tier.check_solution(inst, [1, 3, 0, 9])
+
+
+def test_cli_rejects_dway_with_other_orderings(capsys):
+    from cpim_amd import solve
+    code = solve.main([os.path.join(XML, "test.xml"), "--dway", "--varh", "dom/wdeg"])
+    assert code == 2 and "d-way" in capsys.readouterr().err
