@@ -126,3 +126,15 @@ def test_cli_rejects_dway_with_other_orderings(capsys):
     from cpim_amd import solve
     code = solve.main([os.path.join(XML, "test.xml"), "--dway", "--varh", "dom/wdeg"])
     assert code == 2 and "d-way" in capsys.readouterr().err
+
+
+def test_tier_reports_unreadable_files(tmp_path):
+    # the child process fails on input before any device work
+    from cpim_amd import tier
+    bad = tmp_path / "broken.xml"
+    bad.write_text("<instance><presentation name='x'/><domains nbDomains='1'>")
+    r = tier.run_one(str(bad), timeout_s=20)
+    assert r.status.startswith("ERROR"), r
+    r = tier.run_one(str(tmp_path / "missing.xml"), timeout_s=20)
+    assert r.status.startswith("ERROR"), r
+    assert tier.main([str(bad), "--quiet"]) == 1
