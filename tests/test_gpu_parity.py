@@ -533,6 +533,29 @@ def test_single_large_full_size(gpu):
             assert st2.deletions == 0 and (m.ReadDomains(0) == dom).all()
 
 
+def test_sliced_pass_with_many_variables_per_group(gpu):
+    # 5,000 variables of 16 values: 77,546 tests in about 300 groups of 16
+    # variables each, placed on CUs by their summed degree; a sample of the
+    # outcomes (a fifth of them fail) against the oracle
+    desc = workload.random_binary(5000, 16, 60000, 0.65, 77)
+    o = oracle.Oracle(desc)
+    ok, root, _ = o.root_ac()
+    assert ok
+    tests = all_tests(root, desc)
+    rng = np.random.default_rng(5)
+    pick = np.sort(rng.choice(len(tests), 64, replace=False))
+    want, _ = o.singleton_tests(tests[pick], nthreads=8)
+    assert 0 < int(want.sum()) < 64
+    with GModelAdapter.Build(desc) as m:
+        m.EnforceGAC_Auto()
+        allf, st = m.SingletonTests(tests)
+        assert st.singleton_tests == len(tests)
+        assert (allf[pick] == want).all()
+        # the same tests in reverse order (unsorted input) give the same outcomes
+        rev, _ = m.SingletonTests(tests[::-1].copy())
+        assert (rev[::-1] == allf).all()
+
+
 def test_batched_sac_full_size_sample(gpu):
     desc = workload.batched_sac()
     o = oracle.Oracle(desc)
