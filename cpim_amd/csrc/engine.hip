@@ -439,6 +439,7 @@ struct cpim_gmodel {
   DevAlloc size;   // [max_depth][n]
   DevAlloc front;  // 3 x nfw
   DevAlloc ctl;    // GacCtl
+  GacCtl* ctl_host = nullptr;  // pinned copy of the control block (host rounds)
   DevAlloc block_out;
   DevAlloc async_ctl;
   DevAlloc scratch;  // workgroup-engine scratch when a copy does not fit in LDS
@@ -574,6 +575,7 @@ int cpim_gmodel_destroy(cpim_gmodel* m) {
     DeviceGuard g(m->device);
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     hipStream_t s = m->stream;
+    if (m->ctl_host) (void)hipHostFree(m->ctl_host);
     delete m;
     if (s) (void)hipStreamDestroy(s);
   });
@@ -617,18 +619,16 @@ void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats*
   unsigned* F = m->front.as<unsigned>();
   GacCtl* ctl = m->ctl.as<GacCtl>();
   const int nfw = m->nfw;
-  GacCtl init{};
-  init.last_push_round = -1;
-  init.wipe_round = INT_MAX;
-  HIPC(hipMemcpyAsync(ctl, &init, sizeof(init), hipMemcpyHostToDevice, m->stream));
-  HIPC(hipMemsetAsync(F, 0, sizeof(unsigned) * 3ll * nfw, m->stream));
-  if (assigned_var < 0) {
-    hipLaunchKernelGGL(k_seed_all, dim3((nfw + 255) / 256), dim3(256), 0, m->stream, F, nfw,
-                       m->dm.num_active);
-  } else {
+  if (!m->ctl_host) HIPC(hipHostMalloc((void**)&m->ctl_host, sizeof(GacCtl), hipHostMallocDefault));
+  {
+    const long long words = 3ll * nfw;
+    const int blocks = (int)std::max<long long>(1, std::min<long long>((words + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_gac_init, dim3(blocks), dim3(256), 0, m->stream, ctl, F, nfw, m->dm.num_active,
+                       assigned_var < 0 ? 1 : 0);
+  }
+  if (assigned_var >= 0)
     hipLaunchKernelGGL(k_seed_var, dim3(1), dim3(256), 0, m->stream, F, m->dm.adj_off.as<int>(),
                        m->dm.adj.as<int4>(), assigned_var);
-  }
   HIPC(hipGetLastError());
   DevNet net = m->dm.net();
   const int threads = 256;
@@ -662,8 +662,9 @@ void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats*
       hipLaunchKernelGGL(k_gac_round, dim3(grid), dim3(threads), 0, m->stream, net, dom, size, F, nfw,
                          ctl, r, parts);
       HIPC(hipGetLastError());
-      HIPC(hipMemcpyAsync(&res, ctl, sizeof(res), hipMemcpyDeviceToHost, m->stream));
+      HIPC(hipMemcpyAsync(m->ctl_host, ctl, sizeof(res), hipMemcpyDeviceToHost, m->stream));
       m->sync();
+      res = *m->ctl_host;
       rounds = r + 1;
       if (res.wipe_round <= r) break;
       if (res.last_push_round < r) {

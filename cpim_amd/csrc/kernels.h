@@ -194,12 +194,30 @@ __global__ void k_gac_persistent(DevNet net, u64* dom, int* size, unsigned* F, i
   }
 }
 
-__global__ void k_seed_all(unsigned* F, int nfw, int ncons) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nfw) return;
-  const int lo = (int)(i * 32);
-  const int k = ncons - lo;
-  F[i] = k >= 32 ? 0xffffffffu : (k <= 0 ? 0u : ((1u << k) - 1u));
+// Start of a host-driven fixpoint in one launch: the control block, the three
+// frontier buffers cleared, and with seed_all every constraint in the first.
+__global__ void k_gac_init(GacCtl* ctl, unsigned* F, int nfw, int ncons, int seed_all) {
+  const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i0 == 0) {
+    for (int k = 0; k < 3; ++k) ctl->scanner[k] = 0u;
+    ctl->last_push_round = -1;
+    ctl->wipe_round = INT_MAX;
+    ctl->rounds = 0;
+    ctl->converged = 0;
+    ctl->pad = 0;
+  }
+  if (i0 < kStatSlots) {
+    ctl->deletions[i0] = 0ull;
+    ctl->revisions[i0] = 0ull;
+  }
+  for (long long i = i0; i < 3ll * nfw; i += (long long)gridDim.x * blockDim.x) {
+    unsigned v = 0u;
+    if (seed_all && i < nfw) {
+      const int k = ncons - (int)(i * 32);
+      v = k >= 32 ? 0xffffffffu : (k <= 0 ? 0u : ((1u << k) - 1u));
+    }
+    F[i] = v;
+  }
 }
 
 __global__ void k_seed_var(unsigned* F, const int* adj_off, const int4* adj, int var) {
