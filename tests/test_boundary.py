@@ -169,3 +169,19 @@ def test_tuning_knobs():
         N.tune_set("unit_threads", 100)
     with pytest.raises(N.CpimInvalidArgument, match="unknown tuning knob"):
         N.tune_set("no_such_knob", 1)
+
+
+CLI = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cpim_amd", "cpim_solve")
+
+
+def test_cpp_cli_reports_bad_models_as_invalid_argument():
+    # the C++ facade maps a bad model to std::invalid_argument before any
+    # device work
+    import subprocess
+    if not os.path.exists(CLI):
+        pytest.skip("cpim_solve not built")
+    r = subprocess.run([CLI, "--validate-only"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "invalid_argument: scope names an unknown variable" in r.stdout
+    r = subprocess.run([CLI, "--varh", "wdeg"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "--varh" in r.stderr

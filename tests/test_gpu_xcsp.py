@@ -88,3 +88,47 @@ def test_cli_variable_orderings(gpu, capsys, varh, code):
     m = re.search(r"positives=(\d+), negatives=(\d+)", out)
     assert m and (int(m.group(1)), int(m.group(2))) == (r.num_positive, r.num_negative)
     assert (rc == 0) == bool(r.num_sol)
+
+
+CPP_CLI = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cpim_amd", "cpim_solve")
+
+
+@pytest.mark.parametrize("varh,code", [("dom", 0), ("domdeg", 1), ("domwdeg", 2)])
+@pytest.mark.parametrize("limit", [0, 3000])
+def test_cpp_cli_matches_oracle(gpu, varh, code, limit):
+    # the C++ facade (include/cpim/gmodel.hpp) through cpim_solve: same
+    # generated instance as workload.random_binary, node counts of the oracle
+    import subprocess
+    from cpim_amd import workload
+    if not os.path.exists(CPP_CLI):
+        pytest.skip("cpim_solve not built")
+    args = ["--n", "30", "--d", "16", "--e", "120", "--p", "0.4", "--seed", "11", "--varh", varh]
+    if limit:
+        args += ["--node_limit", str(limit)]
+    r = subprocess.run([CPP_CLI] + args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = re.search(r"positives=(\d+), negatives=(\d+)", r.stdout)
+    desc = workload.random_binary(30, 16, 120, 0.4, 11)
+    with oracle.var_heuristic(code):
+        o = oracle.Oracle(desc).mac(node_limit=limit)
+    assert m and (int(m.group(1)), int(m.group(2))) == (o.num_positive, o.num_negative), r.stdout
+    if o.num_sol:
+        sol = re.search(r"MAC solution \(canonical indices\):((?: \d+)*)", r.stdout)
+        assert sol and [int(x) for x in sol.group(1).split()] == list(o.solution)
+
+
+def test_cpp_cli_sac_and_dway(gpu):
+    import subprocess
+    from cpim_amd import workload
+    if not os.path.exists(CPP_CLI):
+        pytest.skip("cpim_solve not built")
+    base = ["--n", "30", "--d", "16", "--e", "120", "--p", "0.4", "--seed", "11"]
+    r = subprocess.run([CPP_CLI] + base + ["--sac"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    sok, sdom, _ = oracle.Oracle(workload.random_binary(30, 16, 120, 0.4, 11)).sac1()
+    assert ("consistent=true" if sok else "consistent=false") in r.stdout
+    r = subprocess.run([CPP_CLI] + base + ["--dway"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "MAC stats:" in r.stdout, r.stdout + r.stderr
+    r = subprocess.run([CPP_CLI] + base + ["--dway", "--varh", "domdeg"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
