@@ -123,3 +123,23 @@ def test_serving_result_busy_until_done(gpu):
         assert srv.poll() == []
     finally:
         srv.close()
+
+
+def test_batch_with_a_unit_in_global_scratch(gpu):
+    # one unit too large for LDS (3,476 variables) beside small ones: every
+    # unit's root fixpoint and first nodes equal the oracle's
+    from test_gpu_edges import padded_pigeonhole
+    models = [workload.uneven_instance(7), padded_pigeonhole(), workload.uneven_instance(9)]
+    b = Batch(models)
+    try:
+        res, _ = b.run(NODES, trace_capacity=2 * NODES)
+        for i, (m, got) in enumerate(zip(models, res)):
+            ok, root, _ = oracle.Oracle(m).root_ac()
+            r = oracle.Oracle(m).mac(node_limit=NODES, trace_cap=2 * NODES)
+            assert got.root_consistent == ok, i
+            if ok:
+                assert (b.root_domains(i) == root).all(), i
+            assert (got.num_positive, got.num_negative) == (r.num_positive, r.num_negative), i
+            assert got.trace == r.trace, i
+    finally:
+        b.close()
