@@ -322,6 +322,8 @@ struct DevAlloc {
 struct DevModel {
   int n = 0, W = 0, dpad = 0, max_dom = 0, num_active = 0;
   DevAlloc adj_off, adj, con, sup, tsup, cap, nbr;
+  DevAlloc lnet_img;  // the network image for LDS (LdsNet), when small enough
+  LdsNet lnet{};
   long long bytes = 0;
   DevNet net() const {
     DevNet d;
@@ -344,6 +346,64 @@ void upload(DevAlloc& dst, const std::vector<T>& src, hipStream_t s, long long& 
   dst = DevAlloc(b);
   if (!src.empty()) HIPC(hipMemcpyAsync(dst.p, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, s));
   bytes += (long long)b;
+}
+
+// Largest network image copied into LDS by the device MAC (LdsNet).
+constexpr long long kLdsNetMax = 128 * 1024;
+
+long long align16(long long b) { return (b + 15) & ~15ll; }
+
+// The LdsNet image of a network: the arrays of DevNet in one block, planes
+// and tables with value stride dc = max_dom instead of dpad.  Built after the
+// planes (the plane rows are copied from the device planes); skipped when the
+// image would exceed kLdsNetMax.
+void build_lds_image(const ModelImage& im, DevModel& dm, hipStream_t s) {
+  const int n = im.n, W = im.W, dpad = im.dpad, dc = std::max(1, im.max_dom);
+  const long long npairs = (long long)im.bin_cons.size();
+  long long tsup_c = 0;
+  for (const ConMeta& m : im.con)
+    if (m.kind == 1) tsup_c += (long long)m.arity * dc * m.tw;
+  const long long o_adj = align16(4ll * (n + 1));
+  const long long o_con = o_adj + align16(16ll * (long long)im.adj.size());
+  const long long o_sup = o_con + align16((long long)sizeof(ConMeta) * (long long)im.con.size());
+  const long long sup_words = npairs * 2 * W * dc + 64ll * W;
+  const long long o_tsup = o_sup + align16(8 * sup_words);
+  const long long bytes = o_tsup + align16(8 * std::max<long long>(1, tsup_c));
+  if (bytes > kLdsNetMax) return;
+  std::vector<char> h((size_t)bytes, 0);
+  std::memcpy(h.data(), im.adj_off.data(), 4 * (size_t)(n + 1));
+  if (!im.adj.empty()) std::memcpy(h.data() + o_adj, im.adj.data(), 16 * im.adj.size());
+  ConMeta* con = (ConMeta*)(h.data() + o_con);
+  u64* ts = (u64*)(h.data() + o_tsup);
+  long long toff = 0;
+  for (size_t c = 0; c < im.con.size(); ++c) {
+    ConMeta m = im.con[c];
+    if (m.kind == 0) {
+      m.off = m.off / (2ll * W * dpad) * (2ll * W * dc);
+    } else {
+      for (int q = 0; q < m.arity; ++q)
+        for (int a = 0; a < dc; ++a)
+          std::memcpy(ts + toff + ((long long)q * dc + a) * m.tw,
+                      im.tsup.data() + m.off + ((long long)q * dpad + a) * m.tw, 8 * (size_t)m.tw);
+      m.off = toff;
+      toff += (long long)m.arity * dc * m.tw;
+    }
+    con[c] = m;
+  }
+  dm.lnet_img = DevAlloc((size_t)bytes);
+  HIPC(hipMemcpyAsync(dm.lnet_img.p, h.data(), (size_t)bytes, hipMemcpyHostToDevice, s));
+  if (npairs > 0)
+    HIPC(hipMemcpy2DAsync((char*)dm.lnet_img.p + o_sup, 8 * (size_t)dc, dm.sup.p, 8 * (size_t)dpad,
+                          8 * (size_t)dc, (size_t)(npairs * 2 * W), hipMemcpyDeviceToDevice, s));
+  HIPC(hipStreamSynchronize(s));  // h is host stack memory
+  dm.bytes += bytes;
+  dm.lnet.src = dm.lnet_img.as<uint4>();
+  dm.lnet.bytes = (int)bytes;
+  dm.lnet.dc = dc;
+  dm.lnet.o_adj = (int)o_adj;
+  dm.lnet.o_con = (int)o_con;
+  dm.lnet.o_sup = (int)o_sup;
+  dm.lnet.o_tsup = (int)o_tsup;
 }
 
 void build_device_model(const ModelImage& im, DevModel& dm, hipStream_t s) {
@@ -383,6 +443,7 @@ void build_device_model(const ModelImage& im, DevModel& dm, hipStream_t s) {
       HIPC(hipStreamSynchronize(s));
     }
   }
+  build_lds_image(im, dm, s);
   HIPC(hipStreamSynchronize(s));
 }
 
@@ -1097,6 +1158,7 @@ Knob g_knobs[] = {
     {"sac_slice_kernel", "CPIM_SLICE_KERNEL", {0}, false},
     {"unit_threads", "CPIM_UNIT_THREADS", {0}, false},
     {"sac_slice_nibbles", "CPIM_SLICE_NIBBLES", {0}, false},
+    {"mac_lds_net", "CPIM_MAC_LDS_NET", {0}, false},
 };
 std::mutex g_knob_mu;
 
@@ -1105,6 +1167,7 @@ bool knob_valid(const char* name, long long v) {
   if (k == "sac_slice_masks") return v == 0 || v == 1 || v == 2 || v == 4;
   if (k == "sac_slice_kernel") return v == 0 || v == 1;
   if (k == "sac_slice_nibbles") return v == 0 || v == 1 || v == 2;
+  if (k == "mac_lds_net") return v == 0 || v == 1;
   return v == 0 || (v >= 64 && v <= 1024 && v % 64 == 0);
 }
 
@@ -1586,13 +1649,18 @@ int cpim_gmodel_solve(cpim_gmodel* m, const cpim_solve_opts* opts, cpim_search_s
     // host checks the time limit in between.
     A.step_budget = kNodesPerLaunch;
     MacState s{};
+    // The network image goes to LDS too when it fits beside the domains.
+    LdsNet lnet{};
+    if (knob("mac_lds_net") == 0 && m->dm.lnet.bytes > 0 &&
+        bytes + m->dm.lnet.bytes <= (long long)m->props.max_lds - 2048)
+      lnet = m->dm.lnet;
     while (true) {
       if (bytes <= kLdsBudget) {
-        hipLaunchKernelGGL(k_mac<true>, dim3(1), dim3(threads), (size_t)bytes, m->stream, net, A,
-                           (char*)nullptr);
+        hipLaunchKernelGGL(k_mac<true>, dim3(1), dim3(threads), (size_t)(bytes + lnet.bytes), m->stream, net,
+                           A, (char*)nullptr, lnet);
       } else {
         char* scratch = m->get_scratch(bytes);
-        hipLaunchKernelGGL(k_mac<false>, dim3(1), dim3(threads), 0, m->stream, net, A, scratch);
+        hipLaunchKernelGGL(k_mac<false>, dim3(1), dim3(threads), 0, m->stream, net, A, scratch, LdsNet{});
       }
       HIPC(hipGetLastError());
       HIPC(hipMemcpyAsync(&s, state.p, sizeof(s), hipMemcpyDeviceToHost, m->stream));

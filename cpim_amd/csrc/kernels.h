@@ -1516,6 +1516,36 @@ struct NodeRec {
   u64 dom_hash;
 };
 
+// A copy of a network laid out as it sits in LDS behind the working memory:
+// adjacency offsets, adjacency, constraint metadata, support planes, tables.
+// The value stride of the planes and tables is cut from dpad (64 W) to dc,
+// the largest domain, and the constraint offsets follow that stride.  The
+// planes are followed by 64 W words of zeros: a target-side revision loads
+// whole 64-value chunks of a row, and the lanes past dc (values that are
+// never live) must still read inside the image.  bytes = 0: no image.
+struct LdsNet {
+  const uint4* src;  // device copy of the image
+  int bytes;         // multiple of 16
+  int dc;
+  int o_adj, o_con, o_sup, o_tsup;  // byte offsets in the image
+};
+
+// Copies the image behind the working memory (at dst) and returns the
+// network whose arrays are those LDS copies.  Called by every thread.
+__device__ __forceinline__ DevNet wg_load_net(const DevNet& g, const LdsNet& L, char* dst) {
+  uint4* d4 = (uint4*)dst;
+  for (int i = threadIdx.x; i < (L.bytes >> 4); i += blockDim.x) d4[i] = L.src[i];
+  __syncthreads();
+  DevNet ln = g;
+  ln.dpad = L.dc;
+  ln.adj_off = (const int*)dst;
+  ln.adj = (const int4*)(dst + L.o_adj);
+  ln.con = (const ConMeta*)(dst + L.o_con);
+  ln.sup = (const u64*)(dst + L.o_sup);
+  ln.tsup = (const u64*)(dst + L.o_tsup);
+  return ln;
+}
+
 struct MacArgs {
   MacState* state;
   u64* trail;          // [n + 1][n][W]; trail[0] holds the start domains
@@ -1886,13 +1916,19 @@ __device__ __forceinline__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* 
   }
 }
 
+// lnet.bytes > 0 (kLds only): the network image is copied into LDS behind
+// the working memory and the search reads the network from there.
 template <bool kLds>
-__global__ void k_mac(DevNet net, MacArgs args, char* scratch) {
+__global__ void k_mac(DevNet net, MacArgs args, char* scratch, LdsNet lnet) {
   extern __shared__ __align__(16) char smem[];
   __shared__ WgCtl ctl;
-  char* base = kLds ? smem : scratch;
-  WgMem mem = wg_carve(base, net);
-  wg_mac_run(net, mem, &ctl, args);
+  if (kLds && lnet.bytes > 0) {
+    const DevNet ln = wg_load_net(net, lnet, smem + wg_mem_bytes(net.n, net.W));
+    wg_mac_run(ln, wg_carve(smem, ln), &ctl, args);
+  } else {
+    char* base = kLds ? smem : scratch;
+    wg_mac_run(net, wg_carve(base, net), &ctl, args);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1904,8 +1940,9 @@ struct UnitDesc {
   DevNet net;
   MacArgs args;
   long long scratch_off;  // per-unit global scratch when the domain does not fit in LDS
-  int use_lds;
+  int use_lds;            // 1: working memory in LDS; 2: network image in LDS as well
   int pad;
+  LdsNet lnet;
 };
 
 // states_out[u] receives a copy of unit u's state when its launch ends, so
@@ -1922,9 +1959,19 @@ __global__ void k_mac_units(const UnitDesc* __restrict__ units, int count, unsig
     __syncthreads();
     if (u >= count) break;
     const UnitDesc d = units[u];
-    char* base = d.use_lds ? smem : scratch + d.scratch_off;
-    WgMem mem = wg_carve(base, d.net);
-    wg_mac_run(d.net, mem, &ctl, d.args);
+    // One inlined copy of the search per place of the unit's memory (network
+    // image and working memory in LDS, working memory in LDS, global
+    // scratch), so that LDS accesses compile to ds_read / ds_write: one copy
+    // over a pointer that may be either compiles them to flat instructions.
+    const int where = uniform_int(d.use_lds);
+    if (where == 2) {
+      const DevNet ln = wg_load_net(d.net, d.lnet, smem + wg_mem_bytes(d.net.n, d.net.W));
+      wg_mac_run(ln, wg_carve(smem, ln), &ctl, d.args);
+    } else if (where == 1) {
+      wg_mac_run(d.net, wg_carve(smem, d.net), &ctl, d.args);
+    } else {
+      wg_mac_run(d.net, wg_carve(scratch + d.scratch_off, d.net), &ctl, d.args);
+    }
     if (threadIdx.x == 0) states_out[u] = *d.args.state;
     if (only >= 0) break;
   }

@@ -210,8 +210,10 @@ typedef struct cpim_sac_stats {
  *   "sac_slice_nibbles" nibble (four-Russians) tables in the shared stage: 0 = when they fit
  *                       in 80 KB of LDS, 1 = whenever they fit, 2 = never
  *   "unit_threads"      threads per workgroup of the batch / serving MAC kernel
+ *   "mac_lds_net"       device MAC: 0 = copy the network (adjacency, planes, tables) into LDS
+ *                       when it fits, 1 = read it from device memory
  * First values come from CPIM_SLICE_MASKS, CPIM_SLICE_THREADS, CPIM_SLICE_KERNEL,
- * CPIM_SLICE_NIBBLES, CPIM_UNIT_THREADS. */
+ * CPIM_SLICE_NIBBLES, CPIM_UNIT_THREADS, CPIM_MAC_LDS_NET. */
 int cpim_tune_set(const char* name, int64_t value);
 int cpim_tune_get(const char* name, int64_t* value);
 

@@ -155,7 +155,7 @@ def test_serving_request_shape():
 def test_tuning_knobs():
     from cpim_amd import _native as N
     for name in ("sac_slice_masks", "sac_slice_threads", "sac_slice_kernel", "sac_slice_nibbles",
-                 "unit_threads"):
+                 "unit_threads", "mac_lds_net"):
         old = N.tune_get(name)
         N.tune_set(name, 0)
         assert N.tune_get(name) == 0
@@ -167,6 +167,8 @@ def test_tuning_knobs():
         N.tune_set("sac_slice_masks", 3)
     with pytest.raises(N.CpimInvalidArgument, match="out of range"):
         N.tune_set("unit_threads", 100)
+    with pytest.raises(N.CpimInvalidArgument, match="out of range"):
+        N.tune_set("mac_lds_net", 2)
     with pytest.raises(N.CpimInvalidArgument, match="unknown tuning knob"):
         N.tune_set("no_such_knob", 1)
 
