@@ -366,7 +366,9 @@ __device__ __forceinline__ int revise_binary(const DevNet& net, u64* dom, int* s
 // wave) for every variable that lost values.  Any number of tuples and up to
 // CPIM_MAX_ARITY positions.
 //
-// The tuple words are taken in chunks of 64 words (4096 tuples).  In a chunk,
+// One-word domains with at most 512 tuples take a path of their own that
+// revises every position at once (below).  Otherwise the tuple words are
+// taken in chunks of 64 words (4096 tuples).  In a chunk,
 // the OR over live values runs with lanes = (value k, word w) pairs: one round
 // of independent row loads per 64 / Tp values (Tp = chunk words rounded up to
 // a power of two), then an OR across the lanes of each word.  The per-value
@@ -459,7 +461,59 @@ __device__ __forceinline__ void revise_table(const DevNet& net, u64* dom, int* s
       on_removed(y, removed);
     }
   };
-  if (tw <= 64) {
+  if (W == 1 && tw <= 8) {
+    // One-word domains, at most 512 tuples: every position at once.  Lane
+    // (q, w) = (lane / Tp, lane % Tp) ORs word w of the rows of position q's
+    // live values, four loads in flight per step; the AND over positions
+    // leaves valid word w in lane w.  Lane q < arity holds position q's
+    // variable and domain.  Each revised position then tests its values
+    // (lane = value) against the valid words, one row of tw words per lane.
+    const int Tp = pow2_at_least(tw);
+    const int q = lane / Tp, w = lane & (Tp - 1);
+    const bool mine = q < arity && w < tw;
+    const int gv = mine ? mp->scope[q] : 0;
+    const int ov = lane < arity ? mp->scope[lane] : 0;
+    u64 x = mine ? ld_dom<kCoherent>(&dom[gv]) : 0ull;
+    const u64 own = lane < arity ? ld_dom<kCoherent>(&dom[ov]) : 0ull;
+    const u64* tq = ts + (long long)q * dpad * tw + w;
+    u64 acc = 0ull;
+    while (__ballot(x != 0ull)) {
+      int a[4];
+      bool v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        v[i] = x != 0ull;
+        a[i] = v[i] ? (int)__builtin_ctzll(x) : 0;
+        x &= x - 1;
+      }
+      u64 r[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) r[i] = v[i] ? tq[(long long)a[i] * tw] : 0ull;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc |= r[i];
+    }
+    u64 valid = acc;
+    for (int g = 1; g < arity; ++g) valid &= __shfl(acc, g * Tp + w, 64);
+    u64 vw[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) vw[i] = i < tw ? readlane64(valid, i) : 0ull;
+    for (int p = 0; p < arity; ++p) {
+      const int y = __builtin_amdgcn_readlane(ov, p);
+      if (!revised(y)) continue;
+      const u64 dp = readlane64(own, p);
+      bool keep = false;
+      if ((dp >> lane) & 1ull) {
+        const u64* row = ts + ((long long)p * dpad + lane) * tw;
+        u64 r[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r[i] = i < tw ? row[i] : 0ull;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) keep = keep || (r[i] & vw[i]) != 0ull;
+      }
+      const u64 km = __ballot(keep);
+      if (dp & ~km) apply(y, lane == 0 ? dp : 0ull, lane == 0 ? km : 0ull);
+    }
+  } else if (tw <= 64) {
     // one chunk: valid once, then every revised position
     const u64 valid_l = chunk_valid(0, tw);
     for (int p = 0; p < arity; ++p) {
