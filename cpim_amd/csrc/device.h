@@ -84,6 +84,56 @@ __device__ __forceinline__ int wave_inclusive_scan(int v) {
   return v;
 }
 
+// Row operations over DPP (data-parallel primitives: a VALU operand read
+// from another lane of the same 16-lane row), for waves with every lane
+// active.  row_shr:n = 0x110 + n (lanes below n of a row read 0 here),
+// row_ror:n = 0x120 + n.  Each step costs a few cycles where a shuffle
+// through ds_bpermute costs an LDS round trip.
+template <int kCtrl>
+__device__ __forceinline__ unsigned dpp_u32(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, 0xf, 0xf, true);
+}
+
+// OR of v over all 64 lanes, returned to every lane (wave-uniform).
+__device__ __forceinline__ u64 wave_or64_dpp(u64 v) {
+  unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+  lo |= dpp_u32<0x121>(lo);
+  hi |= dpp_u32<0x121>(hi);
+  lo |= dpp_u32<0x122>(lo);
+  hi |= dpp_u32<0x122>(hi);
+  lo |= dpp_u32<0x124>(lo);
+  hi |= dpp_u32<0x124>(hi);
+  lo |= dpp_u32<0x128>(lo);
+  hi |= dpp_u32<0x128>(hi);
+  // every lane of a row now holds the row's OR
+  unsigned rlo = 0, rhi = 0;
+#pragma unroll
+  for (int r = 0; r < 64; r += 16) {
+    rlo |= (unsigned)__builtin_amdgcn_readlane((int)lo, r);
+    rhi |= (unsigned)__builtin_amdgcn_readlane((int)hi, r);
+  }
+  return ((u64)rhi << 32) | rlo;
+}
+
+// Inclusive prefix sum over the 64 lanes; *total receives the sum of all
+// lanes (wave-uniform).
+__device__ __forceinline__ int wave_scan_dpp(int v, int* total) {
+  unsigned x = (unsigned)v;
+  x += dpp_u32<0x111>(x);
+  x += dpp_u32<0x112>(x);
+  x += dpp_u32<0x114>(x);
+  x += dpp_u32<0x118>(x);
+  // lane 16 r + 15 holds the sum of row r
+  const unsigned s0 = (unsigned)__builtin_amdgcn_readlane((int)x, 15);
+  const unsigned s1 = (unsigned)__builtin_amdgcn_readlane((int)x, 31);
+  const unsigned s2 = (unsigned)__builtin_amdgcn_readlane((int)x, 47);
+  const unsigned s3 = (unsigned)__builtin_amdgcn_readlane((int)x, 63);
+  const int row = lane_id() >> 4;
+  x += (row >= 1 ? s0 : 0u) + (row >= 2 ? s1 : 0u) + (row >= 3 ? s2 : 0u);
+  *total = (int)(s0 + s1 + s2 + s3);
+  return (int)x;
+}
+
 // Domain word load.  kCoherent: agent-scope atomic load, which observes every
 // update other workgroups (on any XCD) have completed; used by the
 // asynchronous fixpoint, which has no barrier between rounds.
@@ -167,8 +217,13 @@ __device__ __forceinline__ int revise_binary_at(const DevNet& net, u64* dom, int
       }
       row = plane_t[(long long)w * dpad + a];
     }
-    for (int o = Wp; o < 64; o <<= 1) row |= __shfl_xor(row, o, 64);
-    mask_l = lane < W ? row : 0ull;
+    if (Wp == 1) {
+      const u64 r = wave_or64_dpp(row);
+      mask_l = lane == 0 ? r : 0ull;
+    } else {
+      for (int o = Wp; o < 64; o <<= 1) row |= __shfl_xor(row, o, 64);
+      mask_l = lane < W ? row : 0ull;
+    }
   } else {
     const u64 dt0 = readlane64(dt_l, first_w);
     const u64* plane_u0 = plane_u + (long long)first_w * dpad;
@@ -277,12 +332,20 @@ __device__ __forceinline__ void revise_binary_multi(const DevNet& net, u64* dom,
       for (int p = 0; p < P; ++p)
         row[p] = net.sup[off[p] + (long long)pos_t[p] * W * dpad + (long long)w * dpad + a];
     }
-    for (int o = Wp; o < 64; o <<= 1) {
+    if (Wp == 1) {
 #pragma unroll
-      for (int p = 0; p < P; ++p) row[p] |= __shfl_xor(row[p], o, 64);
+      for (int p = 0; p < P; ++p) {
+        const u64 r = wave_or64_dpp(row[p]);
+        mask_l[p] = lane == 0 ? r : 0ull;
+      }
+    } else {
+      for (int o = Wp; o < 64; o <<= 1) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) row[p] |= __shfl_xor(row[p], o, 64);
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p) mask_l[p] = lane < W ? row[p] : 0ull;
     }
-#pragma unroll
-    for (int p = 0; p < P; ++p) mask_l[p] = lane < W ? row[p] : 0ull;
   } else {
     // target side: lanes are 64 values of u; the word of D(t)'s first
     // non-empty word is loaded for every arc and live chunk before any test

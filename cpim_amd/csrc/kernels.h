@@ -509,7 +509,7 @@ __device__ __forceinline__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl*
         const int v = v0 + lane;
         if (v < n && bit_of(cur, v)) arcs += mem.aoff[v + 1] - mem.aoff[v];
       }
-      for (int d = 32; d > 0; d >>= 1) arcs += __shfl_xor(arcs, d, 64);
+      (void)wave_scan_dpp(arcs, &arcs);
       const int nwaves = nthreads >> 6;
       const int K = uniform_int(max(1, min(kWgArcs, (arcs + 2 * nwaves - 1) / (2 * nwaves))));
       int base = 0, ibase = 0;
@@ -519,13 +519,14 @@ __device__ __forceinline__ bool wg_fixpoint(const DevNet& net, WgMem mem, WgCtl*
         const u64 mask = __ballot(f);
         const int rank = mbcnt(mask);
         const int items = f ? (mem.aoff[v + 1] - mem.aoff[v] + K - 1) / K : 0;
-        const int inc = wave_inclusive_scan(items);
+        int round_items = 0;
+        const int inc = wave_scan_dpp(items, &round_items);
         if (f) {
           mem.list[base + rank] = v;
           mem.pre[base + rank] = ibase + inc - items;
         }
         base += __popcll(mask);
-        ibase += __shfl(inc, 63, 64);
+        ibase += round_items;
       }
       if (lane == 0) {
         mem.pre[base] = ibase;
