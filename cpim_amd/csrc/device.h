@@ -115,6 +115,52 @@ __device__ __forceinline__ u64 wave_or64_dpp(u64 v) {
   return ((u64)rhi << 32) | rlo;
 }
 
+// Minimum and sum of v over all 64 lanes, returned to every lane
+// (wave-uniform).  Rotations by 1, 2, 4 and 8 within a row cover each row
+// exactly once, so every lane ends with its row's result.
+__device__ __forceinline__ u64 dpp_ror_u64_1(u64 v) {
+  return ((u64)dpp_u32<0x121>((unsigned)(v >> 32)) << 32) | dpp_u32<0x121>((unsigned)v);
+}
+__device__ __forceinline__ u64 dpp_ror_u64_2(u64 v) {
+  return ((u64)dpp_u32<0x122>((unsigned)(v >> 32)) << 32) | dpp_u32<0x122>((unsigned)v);
+}
+__device__ __forceinline__ u64 dpp_ror_u64_4(u64 v) {
+  return ((u64)dpp_u32<0x124>((unsigned)(v >> 32)) << 32) | dpp_u32<0x124>((unsigned)v);
+}
+__device__ __forceinline__ u64 dpp_ror_u64_8(u64 v) {
+  return ((u64)dpp_u32<0x128>((unsigned)(v >> 32)) << 32) | dpp_u32<0x128>((unsigned)v);
+}
+
+__device__ __forceinline__ u64 readlane_rows_min(u64 v) {
+  u64 r = readlane64(v, 0);
+#pragma unroll
+  for (int l = 16; l < 64; l += 16) {
+    const u64 o = readlane64(v, l);
+    r = o < r ? o : r;
+  }
+  return r;
+}
+
+__device__ __forceinline__ u64 wave_min64_dpp(u64 v) {
+  u64 o = dpp_ror_u64_1(v);
+  v = o < v ? o : v;
+  o = dpp_ror_u64_2(v);
+  v = o < v ? o : v;
+  o = dpp_ror_u64_4(v);
+  v = o < v ? o : v;
+  o = dpp_ror_u64_8(v);
+  v = o < v ? o : v;
+  return readlane_rows_min(v);
+}
+
+__device__ __forceinline__ u64 wave_sum64_dpp(u64 v) {
+  v += dpp_ror_u64_1(v);
+  v += dpp_ror_u64_2(v);
+  v += dpp_ror_u64_4(v);
+  v += dpp_ror_u64_8(v);
+  return readlane64(v, 0) + readlane64(v, 16) + readlane64(v, 32) + readlane64(v, 48);
+}
+
 // Inclusive prefix sum over the 64 lanes; *total receives the sum of all
 // lanes (wave-uniform).
 __device__ __forceinline__ int wave_scan_dpp(int v, int* total) {

@@ -682,27 +682,25 @@ __device__ __forceinline__ void wg_copy_words(u64* dst, const u64* src, long lon
 }
 
 // Block-wide reduction helpers (all threads call; result valid in every thread).
+// Workgroup reductions, called by every thread: each wave reduces over DPP,
+// lane 0 publishes the wave's result, and every wave reduces the published
+// results (lane i reads wave i's) the same way.
 __device__ __forceinline__ u64 wg_reduce_min_u64(WgCtl* ctl, u64 v) {
-  for (int o = 32; o > 0; o >>= 1) {
-    const u64 other = __shfl_xor(v, o, 64);
-    v = other < v ? other : v;
-  }
-  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  if (lane_id() == 0) ctl->red[wave] = v;
+  v = wave_min64_dpp(v);
+  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, lane = lane_id();
+  if (lane == 0) ctl->red[wave] = v;
   __syncthreads();
-  u64 r = ctl->red[0];
-  for (int i = 1; i < nwaves; ++i) r = ctl->red[i] < r ? ctl->red[i] : r;
+  const u64 r = wave_min64_dpp(lane < nwaves ? ctl->red[lane] : ~0ull);
   __syncthreads();
   return (u64)uniform_ll((long long)r);
 }
 
 __device__ __forceinline__ u64 wg_reduce_sum_u64(WgCtl* ctl, u64 v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  if (lane_id() == 0) ctl->red[wave] = v;
+  v = wave_sum64_dpp(v);
+  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6, lane = lane_id();
+  if (lane == 0) ctl->red[wave] = v;
   __syncthreads();
-  u64 r = 0;
-  for (int i = 0; i < nwaves; ++i) r += ctl->red[i];
+  const u64 r = wave_sum64_dpp(lane < nwaves ? ctl->red[lane] : 0ull);
   __syncthreads();
   return (u64)uniform_ll((long long)r);
 }
