@@ -1462,7 +1462,7 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
       for (int m = 0; m < M; ++m) {
         u64 o = 0ull;
         for (int b = lane; b < dpad; b += 64) o |= ld_dom<true>(&sl[((long long)v * dpad + b) * M + m]);
-        for (int d = 32; d > 0; d >>= 1) o |= __shfl_xor(o, d, 64);
+        o = wave_or64_dpp(o);
         alive_all[m] &= o;
       }
     }
