@@ -115,6 +115,39 @@ __device__ __forceinline__ u64 wave_or64_dpp(u64 v) {
   return ((u64)rhi << 32) | rlo;
 }
 
+// Word-wise OR for W-word rows spread over the wave: lane l holds word
+// l % Wp (Wp = W rounded up to a power of two, at most 16).  Returns, in
+// lane w < W, the OR of word w over all lanes, and 0 in the other lanes.
+// Rotations by Wp, 2 Wp, ..., 8 within a row combine the lanes of one word;
+// then lane w reads its word from each of the four rows.
+__device__ __forceinline__ u64 wave_or_words_dpp(u64 v, int Wp, int W) {
+  unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+  if (Wp <= 1) {
+    lo |= dpp_u32<0x121>(lo);
+    hi |= dpp_u32<0x121>(hi);
+  }
+  if (Wp <= 2) {
+    lo |= dpp_u32<0x122>(lo);
+    hi |= dpp_u32<0x122>(hi);
+  }
+  if (Wp <= 4) {
+    lo |= dpp_u32<0x124>(lo);
+    hi |= dpp_u32<0x124>(hi);
+  }
+  if (Wp <= 8) {
+    lo |= dpp_u32<0x128>(lo);
+    hi |= dpp_u32<0x128>(hi);
+  }
+  const u64 x = ((u64)hi << 32) | lo;
+  const int lane = lane_id();
+  u64 out = 0ull;
+  for (int w = 0; w < W; ++w) {
+    const u64 r = readlane64(x, w) | readlane64(x, 16 + w) | readlane64(x, 32 + w) | readlane64(x, 48 + w);
+    if (lane == w) out = r;
+  }
+  return out;
+}
+
 // Minimum and sum of v over all 64 lanes, returned to every lane
 // (wave-uniform).  Rotations by 1, 2, 4 and 8 within a row cover each row
 // exactly once, so every lane ends with its row's result.
@@ -263,13 +296,7 @@ __device__ __forceinline__ int revise_binary_at(const DevNet& net, u64* dom, int
       }
       row = plane_t[(long long)w * dpad + a];
     }
-    if (Wp == 1) {
-      const u64 r = wave_or64_dpp(row);
-      mask_l = lane == 0 ? r : 0ull;
-    } else {
-      for (int o = Wp; o < 64; o <<= 1) row |= __shfl_xor(row, o, 64);
-      mask_l = lane < W ? row : 0ull;
-    }
+    mask_l = wave_or_words_dpp(row, Wp, W);
   } else {
     const u64 dt0 = readlane64(dt_l, first_w);
     const u64* plane_u0 = plane_u + (long long)first_w * dpad;
@@ -378,20 +405,8 @@ __device__ __forceinline__ void revise_binary_multi(const DevNet& net, u64* dom,
       for (int p = 0; p < P; ++p)
         row[p] = net.sup[off[p] + (long long)pos_t[p] * W * dpad + (long long)w * dpad + a];
     }
-    if (Wp == 1) {
 #pragma unroll
-      for (int p = 0; p < P; ++p) {
-        const u64 r = wave_or64_dpp(row[p]);
-        mask_l[p] = lane == 0 ? r : 0ull;
-      }
-    } else {
-      for (int o = Wp; o < 64; o <<= 1) {
-#pragma unroll
-        for (int p = 0; p < P; ++p) row[p] |= __shfl_xor(row[p], o, 64);
-      }
-#pragma unroll
-      for (int p = 0; p < P; ++p) mask_l[p] = lane < W ? row[p] : 0ull;
-    }
+    for (int p = 0; p < P; ++p) mask_l[p] = wave_or_words_dpp(row[p], Wp, W);
   } else {
     // target side: lanes are 64 values of u; the word of D(t)'s first
     // non-empty word is loaded for every arc and live chunk before any test
