@@ -1840,6 +1840,14 @@ __device__ __forceinline__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* 
       if (dw) a = w * 64 + (int)__builtin_ctzll(dw);
     }
     a = uniform_int(a);
+    if (a < 0) {
+      // x has an empty domain that no revision reached (no constraints, an
+      // empty initial domain): the search ends without a solution, with no
+      // node recorded, as the oracle's restatement does
+      status = MAC_UNSAT;
+      __syncthreads();
+      break;
+    }
     wg_copy_words(A.trail + depth * nw, mem.dom, nw);
     __syncthreads();
     if (threadIdx.x < W) mem.dom[(long long)x * W + threadIdx.x] = (threadIdx.x == (a >> 6)) ? (1ull << (a & 63)) : 0ull;

@@ -744,6 +744,15 @@ int orc_mac(orc_net* net, int64_t time_limit_ms, int64_t node_limit, orc_stats* 
                                         N.n, net->nbr, [&](int v) { return N.size(v, p); },
                                         [&](int v) { return N.assigned(v, p); }, true);
       const int val = N.head(var, p);
+      // An empty domain that no revision reached (a variable without
+      // constraints whose initial domain is empty) is selected first.  The
+      // reference would then call ReduceTo with value -1, an access outside
+      // the domain; here the search ends without a solution instead, and
+      // no node is recorded.
+      if (val < 0) {
+        finished = true;
+        break;
+      }
       N.new_level(p);
       I.push_back({var, val});
       N.assign(var, true, (int)I.size());

@@ -125,3 +125,19 @@ def test_mac_in_global_scratch(gpu, heuristic, varh):
     assert (st.num_solutions, st.num_positive, st.num_negative) == \
         (r.num_sol, r.num_positive, r.num_negative)
     assert trace == r.trace
+
+
+@pytest.mark.parametrize("heuristic,varh", [("dom", 0), ("dom/deg", 1), ("dom/wdeg", 2)])
+def test_mac_stops_at_an_empty_unconstrained_domain(gpu, heuristic, varh):
+    # 53 variables, 20 constraints, a fifth of the values removed: variable
+    # 40 has no constraints and an empty initial domain, which no revision
+    # reaches; the search ends without a solution when it selects it
+    desc = workload.random_binary(53, 3, 20, 0.46, 922837970, preremove=0.2)
+    with oracle.var_heuristic(varh):
+        r = oracle.Oracle(desc, oracle.AC3BIT).mac(node_limit=1500, trace_cap=3000)
+    assert r.num_sol == 0 and not r.time_out
+    with GModelAdapter.Build(desc) as m:
+        st, sol, trace = m.Solve(node_limit=1500, trace_capacity=3000, heuristic=heuristic)
+    assert (st.num_solutions, st.num_positive, st.num_negative, st.time_out) == \
+        (r.num_sol, r.num_positive, r.num_negative, 0)
+    assert trace == r.trace

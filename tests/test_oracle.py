@@ -601,3 +601,24 @@ def test_large_nary_tables_vs_bruteforce(seed):
     with oracle.var_heuristic(2):
         rw = o.mac(node_limit=100, trace_cap=400)
     assert [t[:4] for t in rw.trace] == want and o.weights().tolist() == weight
+
+
+def test_mac_stops_at_an_empty_unconstrained_domain():
+    # variable 2 has no constraints and an empty initial domain: no revision
+    # reaches it, so the root fixpoint holds; the search selects it and ends
+    # without a solution, recording no node for it
+    dom = np.array([3, 3, 3, 3], np.int32)
+    desc = from_tuples(dom, [[0, 1], [1, 3]], [[(0, 1), (1, 2)], [(1, 0), (2, 2)]])
+    init = np.zeros((4, 1), np.uint64)
+    init[:, 0] = 0b111
+    init[2, 0] = 0
+    desc = desc.with_init_dom(init)
+    for varh in (0, 1, 2):
+        with oracle.var_heuristic(varh):
+            r = oracle.Oracle(desc).mac(trace_cap=64)
+        assert r.num_sol == 0 and not r.time_out, varh
+        assert all(rec[1] != 2 for rec in r.trace), varh
+    with oracle.var_heuristic(0):
+        r = oracle.Oracle(desc).mac(trace_cap=64)
+    # dom-min takes the empty domain first: no node at all
+    assert (r.num_positive, r.num_negative, r.trace) == (0, 0, [])
