@@ -356,8 +356,9 @@ long long align16(long long b) { return (b + 15) & ~15ll; }
 // The LdsNet image of a network: the arrays of DevNet in one block, planes
 // and tables with value stride dc = max_dom instead of dpad.  Built after the
 // planes (the plane rows are copied from the device planes); skipped when the
-// image would exceed kLdsNetMax.
-void build_lds_image(const ModelImage& im, DevModel& dm, hipStream_t s) {
+// image would exceed kLdsNetMax.  The copies are queued on s from `h`,
+// which the caller keeps until s is synchronized.
+void build_lds_image(const ModelImage& im, DevModel& dm, hipStream_t s, std::vector<char>& h) {
   const int n = im.n, W = im.W, dpad = im.dpad, dc = std::max(1, im.max_dom);
   const long long npairs = (long long)im.bin_cons.size();
   long long tsup_c = 0;
@@ -370,7 +371,7 @@ void build_lds_image(const ModelImage& im, DevModel& dm, hipStream_t s) {
   const long long o_tsup = o_sup + align16(8 * sup_words);
   const long long bytes = o_tsup + align16(8 * std::max<long long>(1, tsup_c));
   if (bytes > kLdsNetMax) return;
-  std::vector<char> h((size_t)bytes, 0);
+  h.assign((size_t)bytes, 0);
   std::memcpy(h.data(), im.adj_off.data(), 4 * (size_t)(n + 1));
   if (!im.adj.empty()) std::memcpy(h.data() + o_adj, im.adj.data(), 16 * im.adj.size());
   ConMeta* con = (ConMeta*)(h.data() + o_con);
@@ -395,7 +396,6 @@ void build_lds_image(const ModelImage& im, DevModel& dm, hipStream_t s) {
   if (npairs > 0)
     HIPC(hipMemcpy2DAsync((char*)dm.lnet_img.p + o_sup, 8 * (size_t)dc, dm.sup.p, 8 * (size_t)dpad,
                           8 * (size_t)dc, (size_t)(npairs * 2 * W), hipMemcpyDeviceToDevice, s));
-  HIPC(hipStreamSynchronize(s));  // h is host stack memory
   dm.bytes += bytes;
   dm.lnet.src = dm.lnet_img.as<uint4>();
   dm.lnet.bytes = (int)bytes;
@@ -443,7 +443,8 @@ void build_device_model(const ModelImage& im, DevModel& dm, hipStream_t s) {
       HIPC(hipStreamSynchronize(s));
     }
   }
-  build_lds_image(im, dm, s);
+  std::vector<char> lds_image;  // host side of the image copy, until the sync below
+  build_lds_image(im, dm, s, lds_image);
   HIPC(hipStreamSynchronize(s));
 }
 
