@@ -100,6 +100,7 @@ class CGacStats(C.Structure):
         ("inconsistent", C.c_int32),
         ("converged", C.c_int32),
         ("revisions", C.c_int64),
+        ("kernel_ms", C.c_double),
     ]
 
 

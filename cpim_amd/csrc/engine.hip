@@ -510,6 +510,18 @@ struct cpim_gmodel {
   std::vector<int2> assigned;  // d_assigned mirror: (var, value) per level
   DevProps props;
   SacScratch sac;
+  hipEvent_t gev[2] = {nullptr, nullptr};  // kernel timing of the fixpoint entry points
+
+  void gac_events() {
+    for (hipEvent_t& e : gev)
+      if (!e) HIPC(hipEventCreate(&e));
+  }
+  // device time between gev[0] and gev[1] (both recorded, gev[1] complete)
+  double gac_elapsed_ms() {
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, gev[0], gev[1]));
+    return ms;
+  }
 
   u64* level_dom(int level) const { return dom.as<u64>() + (long long)level * dm.n * dm.W; }
   int* level_size(int level) const { return size.as<int>() + (long long)level * dm.n; }
@@ -638,6 +650,8 @@ int cpim_gmodel_destroy(cpim_gmodel* m) {
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     hipStream_t s = m->stream;
     if (m->ctl_host) (void)hipHostFree(m->ctl_host);
+    for (hipEvent_t e : m->gev)
+      if (e) (void)hipEventDestroy(e);
     delete m;
     if (s) (void)hipStreamDestroy(s);
   });
@@ -668,6 +682,29 @@ namespace {
 bool fits_lds(const cpim_gmodel* m) { return wg_mem_bytes(m->dm.n, m->dm.W) <= kLdsBudget; }
 
 
+int knob(const char* name);
+
+// Device wall-clock ticks a one-launch fixpoint (persistent or asynchronous)
+// may run before it reports an error: the fixpoint_budget_us knob, or
+// kAsyncBudgetMs.
+unsigned long long fixpoint_budget_ticks(const cpim_gmodel* m) {
+  const long long khz = std::max<long long>(m->props.wall_khz, 1);
+  const long long us = knob("fixpoint_budget_us");
+  return us > 0 ? (unsigned long long)std::max<long long>(1, khz * us / 1000)
+                : (unsigned long long)(khz * kAsyncBudgetMs);
+}
+
+// Workgroups per CU of the persistent fixpoint (256 threads each).
+constexpr int kPersistentBlocksPerCu = 2;
+
+// Parts per frontier word of a grid round: the smallest power of two (at
+// most 32) that gives about four work units per wave of `waves`.
+int frontier_parts(int nfw, long long waves) {
+  int parts = 1;
+  while (parts < 32 && (long long)nfw * parts < 4 * waves) parts *= 2;
+  return parts;
+}
+
 unsigned long long slot_sum(const unsigned long long (&v)[kStatSlots]) {
   unsigned long long t = 0;
   for (unsigned long long x : v) t += x;
@@ -682,6 +719,9 @@ void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats*
   GacCtl* ctl = m->ctl.as<GacCtl>();
   const int nfw = m->nfw;
   if (!m->ctl_host) HIPC(hipHostMalloc((void**)&m->ctl_host, sizeof(GacCtl), hipHostMallocDefault));
+  m->gac_events();
+  double kms = 0.0;
+  HIPC(hipEventRecord(m->gev[0], m->stream));
   {
     const long long words = 3ll * nfw;
     const int blocks = (int)std::max<long long>(1, std::min<long long>((words + 255) / 256, 1024));
@@ -698,34 +738,46 @@ void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats*
   int rounds = 0;
   GacCtl res{};
   if (persistent) {
+    // Every workgroup must be resident for the grid barrier: at most the
+    // occupancy limit per CU (and at most kPersistentBlocksPerCu) on every CU.
+    // If some were not resident, the barrier's bounded wait turns that into
+    // the time-budget error below.
     int per_cu = 0;
     HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gac_persistent, threads, 0));
-    per_cu = std::max(1, std::min(per_cu, 4));
+    per_cu = std::max(1, std::min(per_cu, kPersistentBlocksPerCu));
     const int waves_per_block = threads / 64;
-    const int want = std::max(1, (nfw + waves_per_block - 1) / waves_per_block);
-    const int grid = std::max(1, std::min(want, per_cu * m->props.cus));
-    int mr = (int)std::min<long long>(max_rounds, INT_MAX);
-    void* args[] = {&net, &dom, &size, &F, (void*)&nfw, &ctl, &mr};
-    HIPC(hipLaunchCooperativeKernel((const void*)k_gac_persistent, dim3(grid), dim3(threads), args, 0,
-                                    m->stream));
-    HIPC(hipMemcpyAsync(&res, ctl, sizeof(res), hipMemcpyDeviceToHost, m->stream));
+    const int parts = frontier_parts(nfw, (long long)per_cu * waves_per_block * m->props.cus);
+    const long long units = (long long)nfw * parts;
+    const int grid = (int)std::max<long long>(
+        1, std::min<long long>((units + waves_per_block - 1) / waves_per_block, (long long)per_cu * m->props.cus));
+    const int mr = (int)std::min<long long>(max_rounds, INT_MAX);
+    const unsigned long long max_ticks = fixpoint_budget_ticks(m);
+    hipLaunchKernelGGL(k_gac_persistent, dim3(grid), dim3(threads), 0, m->stream, net, dom, size, F, nfw, ctl, mr,
+                       parts, max_ticks);
+    HIPC(hipGetLastError());
+    HIPC(hipEventRecord(m->gev[1], m->stream));
+    HIPC(hipMemcpyAsync(m->ctl_host, ctl, sizeof(res), hipMemcpyDeviceToHost, m->stream));
     m->sync();
+    kms = m->gac_elapsed_ms();
+    res = *m->ctl_host;
+    if (res.err) fail(CPIM_ERR_RUNTIME, "persistent fixpoint exceeded its time budget at a grid barrier");
     rounds = res.rounds;
   } else {
     // about four work units per resident wave (32 per CU)
-    const long long target = 4ll * 32 * m->props.cus;
-    int parts = 1;
-    while (parts < 32 && (long long)nfw * parts < target) parts *= 2;
+    const int parts = frontier_parts(nfw, 32ll * m->props.cus);
     const int waves_per_block = threads / 64;
     const long long units = (long long)nfw * parts;
     const int grid = (int)std::max<long long>(1, std::min<long long>((units + waves_per_block - 1) / waves_per_block,
                                                                       8ll * m->props.cus));
     for (int r = 0; r < max_rounds; ++r) {
+      if (r > 0) HIPC(hipEventRecord(m->gev[0], m->stream));
       hipLaunchKernelGGL(k_gac_round, dim3(grid), dim3(threads), 0, m->stream, net, dom, size, F, nfw,
                          ctl, r, parts);
       HIPC(hipGetLastError());
+      HIPC(hipEventRecord(m->gev[1], m->stream));
       HIPC(hipMemcpyAsync(m->ctl_host, ctl, sizeof(res), hipMemcpyDeviceToHost, m->stream));
       m->sync();
+      kms += m->gac_elapsed_ms();
       res = *m->ctl_host;
       rounds = r + 1;
       if (res.wipe_round <= r) break;
@@ -740,6 +792,7 @@ void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats*
   st->inconsistent = res.wipe_round != INT_MAX;
   st->converged = st->inconsistent ? 0 : res.converged;
   st->revisions = (int64_t)slot_sum(res.revisions);
+  st->kernel_ms = kms;
 }
 
 void gac_async(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
@@ -750,6 +803,8 @@ void gac_async(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
   const int nfw = m->nfw;
   if (!m->async_ctl.p) m->async_ctl = DevAlloc(sizeof(AsyncCtl));
   AsyncCtl* ctl = m->async_ctl.as<AsyncCtl>();
+  m->gac_events();
+  HIPC(hipEventRecord(m->gev[0], m->stream));
   AsyncCtl init{};
   HIPC(hipMemcpyAsync(ctl, &init, sizeof(init), hipMemcpyHostToDevice, m->stream));
   HIPC(hipMemsetAsync(F, 0, sizeof(unsigned) * nfw, m->stream));
@@ -759,14 +814,15 @@ void gac_async(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
   const int threads = 256;
   const int waves = std::max(1, std::min(nfw, 4 * m->props.cus));
   const int blocks = (waves + threads / 64 - 1) / (threads / 64);
-  // time budget of one launch: kAsyncBudgetMs of device wall clock
-  const unsigned long long max_ticks = (unsigned long long)std::max<long long>(m->props.wall_khz, 1) * kAsyncBudgetMs;
+  const unsigned long long max_ticks = fixpoint_budget_ticks(m);
   DevNet net = m->dm.net();
   hipLaunchKernelGGL(k_gac_async, dim3(blocks), dim3(threads), 0, m->stream, net, dom, size, F, nfw, ctl, max_ticks);
   HIPC(hipGetLastError());
+  HIPC(hipEventRecord(m->gev[1], m->stream));
   AsyncCtl res{};
   HIPC(hipMemcpyAsync(&res, ctl, sizeof(res), hipMemcpyDeviceToHost, m->stream));
   m->sync();
+  st->kernel_ms = m->gac_elapsed_ms();
   if (res.err) fail(CPIM_ERR_RUNTIME, "asynchronous fixpoint exceeded its time budget");
   st->iterations = 1;
   st->deletions = (int32_t)slot_sum(res.deletions);
@@ -781,6 +837,8 @@ void gac_block(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
   const long long bytes = wg_mem_bytes(m->dm.n, m->dm.W);
   const int threads = 1024;
   BlockGacOut* out = m->block_out.as<BlockGacOut>();
+  m->gac_events();
+  HIPC(hipEventRecord(m->gev[0], m->stream));
   if (bytes <= kLdsBudget) {
     hipLaunchKernelGGL(k_block_gac<true>, dim3(1), dim3(threads), (size_t)bytes, m->stream, net,
                        m->level_dom(L), m->level_size(L), assigned_var, (char*)nullptr, out);
@@ -790,9 +848,11 @@ void gac_block(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
                        m->level_size(L), assigned_var, scratch, out);
   }
   HIPC(hipGetLastError());
+  HIPC(hipEventRecord(m->gev[1], m->stream));
   BlockGacOut res{};
   HIPC(hipMemcpyAsync(&res, out, sizeof(res), hipMemcpyDeviceToHost, m->stream));
   m->sync();
+  st->kernel_ms = m->gac_elapsed_ms();
   st->iterations = 1;
   st->deletions = (int32_t)res.deletions;
   st->inconsistent = !res.consistent;
@@ -814,9 +874,6 @@ int auto_gac_mode(const cpim_gmodel* m, int assigned_var) {
 void gac(cpim_gmodel* m, int assigned_var, int mode, cpim_gac_stats* st) {
   std::memset(st, 0, sizeof(*st));
   if (mode == CPIM_GAC_AUTO) mode = auto_gac_mode(m, assigned_var);
-  if (mode == CPIM_GAC_PERSISTENT)
-    fail(CPIM_ERR_UNSUPPORTED,
-         "persistent GAC is switched off: its first GPU run went silent and is under investigation");
   if (m->dm.num_active == 0) {
     st->converged = 1;
     return;
@@ -1160,6 +1217,7 @@ Knob g_knobs[] = {
     {"unit_threads", "CPIM_UNIT_THREADS", {0}, false},
     {"sac_slice_nibbles", "CPIM_SLICE_NIBBLES", {0}, false},
     {"mac_lds_net", "CPIM_MAC_LDS_NET", {0}, false},
+    {"fixpoint_budget_us", "CPIM_FIXPOINT_BUDGET_US", {0}, false},
 };
 std::mutex g_knob_mu;
 
@@ -1169,6 +1227,7 @@ bool knob_valid(const char* name, long long v) {
   if (k == "sac_slice_kernel") return v == 0 || v == 1;
   if (k == "sac_slice_nibbles") return v == 0 || v == 1 || v == 2;
   if (k == "mac_lds_net") return v == 0 || v == 1;
+  if (k == "fixpoint_budget_us") return v >= 0 && v <= 600000000ll;
   return v == 0 || (v >= 64 && v <= 1024 && v % 64 == 0);
 }
 

@@ -4,11 +4,7 @@
 // instances and the serving step.
 #pragma once
 
-#include <hip/hip_cooperative_groups.h>
-
 #include "device.h"
-
-namespace cg = cooperative_groups;
 
 // ===========================================================================
 // Model build: dense rows -> word-major support planes (both directions).
@@ -73,6 +69,12 @@ struct GacCtl {
   int rounds;           // rounds executed (written at the end)
   int converged;
   int pad;
+  // grid barrier of the persistent kernel: arrivals in this generation, the
+  // generation, and the error flag set by a block whose wait ran out of time
+  unsigned bar_count;
+  unsigned bar_gen;
+  int err;
+  int pad2;
   unsigned long long deletions[kStatSlots];
   unsigned long long revisions[kStatSlots];
 };
@@ -92,6 +94,7 @@ __device__ __forceinline__ void push_var(const DevNet& net, int v, unsigned* B, 
   if (lane == 0 && ld_agent(&ctl->last_push_round) < r) atomicMax(&ctl->last_push_round, r);
 }
 
+template <bool kCoherent = false>
 __device__ __forceinline__ void grid_process_constraint(const DevNet& net, u64* dom, int* size,
                                                         int c, unsigned* B, GacCtl* ctl, int r,
                                                         unsigned long long& revs,
@@ -100,14 +103,14 @@ __device__ __forceinline__ void grid_process_constraint(const DevNet& net, u64* 
   bool wiped = false;
   if (mp->kind == 0) {
     const int x = mp->scope[0], y = mp->scope[1];
-    int rx = revise_binary(net, dom, size, c, x, y, 1, &wiped);
+    int rx = revise_binary<kCoherent>(net, dom, size, c, x, y, 1, &wiped);
     ++revs;
     if (wiped) atomicMin(&ctl->wipe_round, r);
     if (rx) {
       dels += rx;
       push_var(net, x, B, ctl, r);
     }
-    int ry = revise_binary(net, dom, size, c, y, x, 0, &wiped);
+    int ry = revise_binary<kCoherent>(net, dom, size, c, y, x, 0, &wiped);
     ++revs;
     if (wiped) atomicMin(&ctl->wipe_round, r);
     if (ry) {
@@ -115,12 +118,11 @@ __device__ __forceinline__ void grid_process_constraint(const DevNet& net, u64* 
       push_var(net, y, B, ctl, r);
     }
   } else {
-    revise_table(net, dom, size, c, -1, nullptr,
-                 [&](int v, int k) {
-                   dels += k;
-                   push_var(net, v, B, ctl, r);
-                 },
-                 &wiped);
+    auto on_removed = [&](int v, int k) {
+      dels += k;
+      push_var(net, v, B, ctl, r);
+    };
+    revise_table<decltype(on_removed), kCoherent>(net, dom, size, c, -1, nullptr, on_removed, &wiped);
     ++revs;
     if (wiped) atomicMin(&ctl->wipe_round, r);
   }
@@ -129,7 +131,9 @@ __device__ __forceinline__ void grid_process_constraint(const DevNet& net, u64* 
 // Work units are (frontier word, part) pairs: part p of word w covers the
 // constraint bits [p * 32 / parts, (p + 1) * 32 / parts) of A[w], so a round
 // over few words still spreads over many waves; waves take units in a fixed
-// stride.
+// stride.  kCoherent: domains and frontier words are read with agent-scope
+// loads (rounds of one launch, written by other XCDs since the last barrier).
+template <bool kCoherent = false>
 __device__ __forceinline__ void grid_round(const DevNet& net, u64* dom, int* size, unsigned* F,
                                            int nfw, GacCtl* ctl, int r, int parts) {
   unsigned* A = F + (long long)(r % 3) * nfw;
@@ -137,8 +141,10 @@ __device__ __forceinline__ void grid_round(const DevNet& net, u64* dom, int* siz
   unsigned* Cl = F + (long long)((r + 2) % 3) * nfw;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(&ctl->scanner[(r + 1) % 3], 0u);
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nfw;
-       i += (long long)gridDim.x * blockDim.x)
-    Cl[i] = 0u;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (kCoherent) __hip_atomic_store(&Cl[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else Cl[i] = 0u;
+  }
   const int lane = lane_id();
   unsigned long long revs = 0, dels = 0;
   const long long units = (long long)nfw * parts;
@@ -152,11 +158,12 @@ __device__ __forceinline__ void grid_round(const DevNet& net, u64* dom, int* siz
     const int wi = (int)(ui / parts);
     const int part = ui - wi * parts;
     const unsigned pmask = pbits == 32 ? ~0u : (((1u << pbits) - 1u) << (part * pbits));
-    unsigned bits = A[wi] & pmask;
+    const unsigned aw = kCoherent ? (unsigned)ld_agent((const int*)&A[wi]) : A[wi];
+    unsigned bits = aw & pmask;
     while (bits) {
       const int b = __ffs(bits) - 1;
       bits &= bits - 1u;
-      grid_process_constraint(net, dom, size, wi * 32 + b, B, ctl, r, revs, dels);
+      grid_process_constraint<kCoherent>(net, dom, size, wi * 32 + b, B, ctl, r, revs, dels);
     }
   }
   if (lane == 0 && (revs | dels)) {
@@ -171,26 +178,83 @@ __global__ void k_gac_round(DevNet net, u64* dom, int* size, unsigned* F, int nf
   grid_round(net, dom, size, F, nfw, ctl, r, parts);
 }
 
-// Every round in one cooperative launch.
-__global__ void k_gac_persistent(DevNet net, u64* dom, int* size, unsigned* F, int nfw,
-                                 GacCtl* ctl, int max_rounds) {
-  cg::grid_group grid = cg::this_grid();
-  int r = 0;
-  int converged = 0;
-  for (; r < max_rounds; ++r) {
-    grid_round(net, dom, size, F, nfw, ctl, r, 1);
-    grid.sync();
-    const int wr = ld_agent(&ctl->wipe_round);
-    const int lp = ld_agent(&ctl->last_push_round);
-    if (wr <= r) break;
-    if (lp < r) {
-      converged = 1;
-      break;
+// ===========================================================================
+// Persistent fixpoint: every round in one launch, rounds separated by a grid
+// barrier.  The grid is sized by the host so that every workgroup is resident
+// at once (occupancy x CUs).
+//
+// The barrier is a generation counter: thread 0 of each workgroup reads the
+// generation (acquire), then adds itself to the arrivals; the last arrival
+// resets the count and bumps the generation, the others wait for the bump.
+// Every wait is bounded: a waiter whose device wall clock passes the launch's
+// budget, or that sees the error flag another block raised, stops waiting and
+// its block leaves, so the launch always ends (an error, never a hang), even
+// if the workgroups were not all resident.  The decision taken after the
+// barrier (next round, wipe-out, fixpoint, error) is made by thread 0 and
+// broadcast through LDS; every wave reads it back as a wave-uniform value
+// (readfirstlane), so the round loop leaves through a scalar branch.
+// The round decision uses epochs that only grow (last_push_round,
+// wipe_round): a block that already runs round r + 1 can raise them, but
+// never so that a slower block would decide differently for round r.
+// ===========================================================================
+enum : int { kRoundNext = 0, kRoundWipe = 1, kRoundFixpoint = 2, kRoundError = 3 };
+
+__device__ __forceinline__ int grid_barrier_decide(GacCtl* ctl, unsigned nblocks, int r,
+                                                   unsigned long long t0, unsigned long long max_ticks,
+                                                   int* s_decision) {
+  __threadfence();  // release: this wave's removals, pushes and epochs
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived = atomicAdd(&ctl->bar_count, 1u) + 1u;
+    int st = kRoundNext;
+    if (arrived == nblocks) {
+      atomicExch(&ctl->bar_count, 0u);
+      __threadfence();
+      atomicAdd(&ctl->bar_gen, 1u);
+    } else {
+      while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        if (__hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          st = kRoundError;
+          break;
+        }
+        if (wall_clock64() - t0 > max_ticks) {
+          atomicExch(&ctl->err, 1);
+          st = kRoundError;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
     }
+    __threadfence();  // acquire: every block's round
+    if (st == kRoundNext && __hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      st = kRoundError;
+    if (st == kRoundNext) {
+      const int wr = ld_agent(&ctl->wipe_round);
+      const int lp = ld_agent(&ctl->last_push_round);
+      st = wr <= r ? kRoundWipe : (lp < r ? kRoundFixpoint : kRoundNext);
+    }
+    *s_decision = st;
+  }
+  __syncthreads();
+  return uniform_int(*s_decision);
+}
+
+__global__ void __launch_bounds__(256) k_gac_persistent(DevNet net, u64* dom, int* size, unsigned* F, int nfw,
+                                                        GacCtl* ctl, int max_rounds, int parts,
+                                                        unsigned long long max_ticks) {
+  __shared__ int s_decision;
+  const unsigned long long t0 = wall_clock64();
+  int r = 0;
+  int decision = kRoundNext;
+  for (; r < max_rounds; ++r) {
+    grid_round<true>(net, dom, size, F, nfw, ctl, r, parts);
+    decision = grid_barrier_decide(ctl, gridDim.x, r, t0, max_ticks, &s_decision);
+    if (decision != kRoundNext) break;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->rounds = r < max_rounds ? r + 1 : max_rounds;
-    ctl->converged = converged;
+    ctl->converged = decision == kRoundFixpoint ? 1 : 0;
   }
 }
 
@@ -205,6 +269,10 @@ __global__ void k_gac_init(GacCtl* ctl, unsigned* F, int nfw, int ncons, int see
     ctl->rounds = 0;
     ctl->converged = 0;
     ctl->pad = 0;
+    ctl->bar_count = 0u;
+    ctl->bar_gen = 0u;
+    ctl->err = 0;
+    ctl->pad2 = 0;
   }
   if (i0 < kStatSlots) {
     ctl->deletions[i0] = 0ull;

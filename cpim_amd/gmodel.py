@@ -69,6 +69,7 @@ class GacStats:
     inconsistent: bool = False
     converged: bool = False
     revisions: int = 0
+    kernel_ms: float = 0.0  # device time of the fixpoint's kernels
 
 
 @dataclass
@@ -145,7 +146,7 @@ class GModel:
         st = N.CGacStats()
         N.check(N.gpu_lib().cpim_gmodel_enforce_gac(self.handle, assigned_var, mode, C.byref(st)))
         return GacStats(st.iterations, st.deletions, bool(st.inconsistent), bool(st.converged),
-                        st.revisions)
+                        st.revisions, st.kernel_ms)
 
     def EnforceGAC(self, verbose: bool = True, assigned_var: int = -1) -> GacStats:
         """Host-driven rounds (one launch + sync per round), like the reference."""

@@ -90,6 +90,29 @@ def batched_sac(index: int = 0, n: int = 500, d: int = 256, e: int = 20000,
     return random_binary(n, d, e, p, instance_seed(3, index), name=f"c3-{index}")
 
 
+# Tightness of the cascade variants of the two large shapes.  At t=0.25 the
+# single fixpoint deletes nothing and at t=0.5 every singleton test survives;
+# these tightnesses were found with the oracle so that the same shapes exercise
+# long propagation chains and failures.
+CASCADE_FIXPOINT_TIGHTNESS = 0.892
+CASCADE_SAC_TIGHTNESS = 0.86
+
+
+def single_large_cascade(index: int = 1) -> ModelDesc:
+    """n=2000, d=64, e=40000, t=0.892, 10% of values pre-removed: the root
+    fixpoint runs 25 frontier rounds and deletes 13% of the values (15,369 of
+    115,143) without a wipe-out (index 1)."""
+    return random_binary(2000, 64, 40000, CASCADE_FIXPOINT_TIGHTNESS, instance_seed(2, index),
+                         preremove=0.10, name=f"c2-cascade-{index}")
+
+
+def batched_sac_cascade(index: int = 1) -> ModelDesc:
+    """n=500, d=256, e=20000, t=0.86: about 42% of the 128,000 singleton tests
+    wipe out after long propagation chains (index 1)."""
+    return random_binary(500, 256, 20000, CASCADE_SAC_TIGHTNESS, instance_seed(3, index),
+                         name=f"c3-cascade-{index}")
+
+
 UNEVEN_DOMAINS = (8, 16, 32, 64, 128)
 
 

@@ -138,18 +138,23 @@ typedef struct cpim_gmodel_info {
 int cpim_gmodel_get_info(const cpim_gmodel* model, cpim_gmodel_info* out);
 
 /* GacStats (include/GModel.cuh:17-21).  deletions is the popcount delta of
- * the level (never double counted).  revisions counts executed arc revisions. */
+ * the level (never double counted).  revisions counts executed arc revisions.
+ * kernel_ms is the device time of the fixpoint's kernels, measured with HIP
+ * events on the model's stream (summed over the rounds of host-driven rounds,
+ * so the host round trips between them are not in it). */
 typedef struct cpim_gac_stats {
   int32_t iterations;
   int32_t deletions;
   int32_t inconsistent;
   int32_t converged;
   int64_t revisions;
+  double kernel_ms;
 } cpim_gac_stats;
 
 #define CPIM_GAC_PLAIN 0      /* host-driven rounds, one launch + sync per round */
-#define CPIM_GAC_PERSISTENT 1 /* all rounds in one cooperative launch (grid barrier);
-                                 currently switched off: returns CPIM_ERR_UNSUPPORTED */
+#define CPIM_GAC_PERSISTENT 1 /* all rounds in one launch of co-resident workgroups, a grid
+                                 barrier between rounds; every barrier wait is bounded by the
+                                 fixpoint_budget_us knob (CPIM_ERR_RUNTIME when it runs out) */
 #define CPIM_GAC_BLOCK 2      /* one workgroup, domains in LDS, one launch */
 #define CPIM_GAC_AUTO 3       /* autotuned choice among the modes that run */
 #define CPIM_GAC_ASYNC 4      /* one launch, whole GPU, asynchronous work queue,
@@ -212,8 +217,11 @@ typedef struct cpim_sac_stats {
  *   "unit_threads"      threads per workgroup of the batch / serving MAC kernel
  *   "mac_lds_net"       device MAC: 0 = copy the network (adjacency, planes, tables) into LDS
  *                       when it fits, 1 = read it from device memory
+ *   "fixpoint_budget_us" device wall-clock budget of one persistent or asynchronous fixpoint
+ *                       launch in microseconds (0 = 20 s); a launch that exceeds it stops and
+ *                       the call returns CPIM_ERR_RUNTIME
  * First values come from CPIM_SLICE_MASKS, CPIM_SLICE_THREADS, CPIM_SLICE_KERNEL,
- * CPIM_SLICE_NIBBLES, CPIM_UNIT_THREADS, CPIM_MAC_LDS_NET. */
+ * CPIM_SLICE_NIBBLES, CPIM_UNIT_THREADS, CPIM_MAC_LDS_NET, CPIM_FIXPOINT_BUDGET_US. */
 int cpim_tune_set(const char* name, int64_t value);
 int cpim_tune_get(const char* name, int64_t* value);
 

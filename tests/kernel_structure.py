@@ -1,21 +1,29 @@
-"""Structure check of the compiled gfx950 kernels (CPU only).
+"""Structure checks of the compiled gfx950 kernels (CPU only).
 
-A loop that contains s_barrier must leave through a uniform (scalar) branch.
-If the compiler lowers it into an exec-mask loop (s_andn2_b64 exec, exec, ...;
-s_cbranch_execz), some lanes can keep iterating past a barrier while another
-lane of the same wave waits outside the loop: a workgroup that publishes its
-next work item from one thread then repeats the same item forever.  This
-module parses the AMD assembly built from cpim_amd/csrc/engine.hip and lists
-such loops per kernel.
+Two properties of the AMD assembly built from cpim_amd/csrc/engine.hip:
+
+* A loop that contains s_barrier must leave through a uniform (scalar)
+  branch.  If the compiler lowers it into an exec-mask loop
+  (s_andn2_b64 exec, exec, ...; s_cbranch_execz), some lanes can keep
+  iterating past a barrier while another lane of the same wave waits outside
+  the loop: a workgroup that publishes its next work item from one thread then
+  repeats the same item forever.
+* A loop that sleeps (s_sleep, i.e. a spin-wait on another workgroup) must
+  read the device wall clock (s_memrealtime / s_memtime) on every iteration:
+  that is the bounded wait that turns a missing partner into an error flag
+  instead of a hang.
+
+Loops are recovered from the compiler's block comments ("Loop Header",
+"Header=BBx_y", "Parent Loop BBx_y"): every block is assigned to every loop
+that encloses it.
 """
 import re
 
 
-def barrier_loops_with_exec_exit(asm_text):
-    text = asm_text.split("\n")
+def _kernels(asm_text):
     kernels = {}
     cur = None
-    for line in text:
+    for line in asm_text.split("\n"):
         m = re.match(r"^(_Z\w+):", line)
         if m:
             cur = m.group(1)
@@ -24,8 +32,13 @@ def barrier_loops_with_exec_exit(asm_text):
             kernels[cur].append(line)
             if "s_endpgm" in line:
                 cur = None
+    return kernels
+
+
+def kernel_loops(asm_text):
+    """{kernel: {loop header: [(block label, [lines])]}} over every loop."""
     result = {}
-    for name, lines in kernels.items():
+    for name, lines in _kernels(asm_text).items():
         blocks = []
         label, chain, body = ".entry", [], []
         for ln in lines:
@@ -45,9 +58,27 @@ def barrier_loops_with_exec_exit(asm_text):
         for lab, ch, b in blocks:
             for h in set(ch):
                 loops.setdefault(h, []).append((lab, b))
+        result[name] = loops
+    return result
+
+
+def _has(blocks, pattern):
+    return any(re.search(pattern, l) for _, b in blocks for l in b)
+
+
+def barrier_loops(asm_text):
+    """{kernel: sorted headers of the loops that contain s_barrier}."""
+    return {k: sorted(h for h, bl in loops.items() if _has(bl, r"\bs_barrier\b"))
+            for k, loops in kernel_loops(asm_text).items()}
+
+
+def barrier_loops_with_exec_exit(asm_text):
+    """{kernel: sorted headers of barrier loops left through an exec-mask branch}."""
+    result = {}
+    for name, loops in kernel_loops(asm_text).items():
         bad = set()
         for h, bl in loops.items():
-            if not any("s_barrier" in l for _, b in bl for l in b):
+            if not _has(bl, r"\bs_barrier\b"):
                 continue
             labels = {lab[2:] if lab.startswith(".L") else lab for lab, _ in bl}
             for _, b in bl:
@@ -58,3 +89,23 @@ def barrier_loops_with_exec_exit(asm_text):
                             bad.add(h)
         result[name] = sorted(bad)
     return result
+
+
+def unbounded_spin_loops(asm_text):
+    """{kernel: sorted headers of loops that sleep without reading the wall clock}.
+    Only innermost sleeping loops matter: an outer loop around a bounded spin
+    contains the clock read of that spin."""
+    result = {}
+    for name, loops in kernel_loops(asm_text).items():
+        bad = []
+        for h, bl in loops.items():
+            if _has(bl, r"\bs_sleep\b") and not _has(bl, r"\bs_mem(real)?time\b"):
+                bad.append(h)
+        result[name] = sorted(bad)
+    return result
+
+
+def sleeping_loops(asm_text):
+    """{kernel: sorted headers of the loops that contain s_sleep}."""
+    return {k: sorted(h for h, bl in loops.items() if _has(bl, r"\bs_sleep\b"))
+            for k, loops in kernel_loops(asm_text).items()}

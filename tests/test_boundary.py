@@ -35,7 +35,7 @@ def test_ctypes_signatures_cover_the_header():
 def test_struct_layouts_match_header_sizes():
     # 64-bit ABI sizes of the plain structs (alignment as in the header)
     assert C.sizeof(N.CModelDesc) == 80
-    assert C.sizeof(N.CGacStats) == 24
+    assert C.sizeof(N.CGacStats) == 32
     assert C.sizeof(N.CNodeRecord) == 24
     assert C.sizeof(N.CSearchStats) == 64
     assert C.sizeof(N.CBatchResult) == 32

@@ -19,16 +19,8 @@ from cpim_amd.model import domains_to_sets, from_tuples
 
 pytestmark = pytest.mark.gpu
 
-# The persistent (cooperative) mode is switched off in the engine: its first
-# GPU run went silent and the cause is not established yet.
-GAC_MODES = {"plain": N.CPIM_GAC_PLAIN, "block": N.CPIM_GAC_BLOCK, "async": N.CPIM_GAC_ASYNC}
-
-
-def test_persistent_mode_reports_switched_off(gpu):
-    desc = workload.random_binary(10, 4, 12, 0.3, 1)
-    with GModelAdapter.Build(desc) as m:
-        with pytest.raises(N.CpimError, match="switched off"):
-            m._gac(-1, N.CPIM_GAC_PERSISTENT)
+GAC_MODES = {"plain": N.CPIM_GAC_PLAIN, "persistent": N.CPIM_GAC_PERSISTENT,
+             "block": N.CPIM_GAC_BLOCK, "async": N.CPIM_GAC_ASYNC}
 
 
 def popcount(dom):

@@ -13,7 +13,8 @@ from cpim_amd.model import from_tuples
 
 pytestmark = pytest.mark.gpu
 
-GAC_MODES = {"plain": N.CPIM_GAC_PLAIN, "block": N.CPIM_GAC_BLOCK, "async": N.CPIM_GAC_ASYNC}
+GAC_MODES = {"plain": N.CPIM_GAC_PLAIN, "persistent": N.CPIM_GAC_PERSISTENT, "block": N.CPIM_GAC_BLOCK,
+             "async": N.CPIM_GAC_ASYNC}
 CASES = 40
 
 
