@@ -12,7 +12,8 @@ on bounded samples of the same tests in the same run.
 Prints one JSON line.  roofline.achieved counts the schedule-independent
 algorithmic bytes of each test's seed revisions (8 W (4 deg(x) + n) per test);
 roofline.traffic is the measured HBM bytes per launch from the committed
-rocprofv3 counter summary under profiles/.
+rocprofv3 counter summary under profiles/, reported only when that summary
+was taken from the kernel sources in the tree.
 """
 from __future__ import annotations
 
@@ -63,20 +64,43 @@ def parse():
 PROFILES = {"sliced": os.path.join(REPO, "profiles", "k_sliced_wide_batched_sac.txt")}
 
 
-def profiled_traffic(mode: str) -> float | None:
-    """HBM bytes per launch of the pass kernel on this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel calibration)."""
+def kernel_source_fingerprint() -> str:
+    """sha256 over the engine's device and launch sources (cpim_amd/csrc), in
+    name order: a committed profile taken from other sources is stale."""
+    import glob
+    import hashlib
+    h = hashlib.sha256()
+    for path in sorted(glob.glob(os.path.join(REPO, "cpim_amd", "csrc", "*"))):
+        if path.endswith((".h", ".hip", ".inc")):
+            h.update(os.path.basename(path).encode())
+            with open(path, "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()
+
+
+def profiled_traffic(mode: str):
+    """(HBM bytes per launch of the pass kernel, note), from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
+    calibration).  The bytes are None when the summary was taken from other
+    kernel sources than the ones in the tree (its kernel_sources_sha256 line)."""
     path = PROFILES.get(mode)
     if path is None:
-        return None
+        return None, "no counter profile for this pass kernel"
+    traffic, stamp = None, None
     try:
         with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                    traffic = float(line.split(":", 1)[1])
+                elif line.startswith("kernel_sources_sha256:"):
+                    stamp = line.split(":", 1)[1].strip()
     except OSError:
-        pass
-    return None
+        return None, f"{os.path.basename(path)} missing"
+    now = kernel_source_fingerprint()
+    if stamp != now:
+        return None, (f"{os.path.relpath(path, REPO)} was taken from other kernel sources "
+                      f"({(stamp or 'unstamped')[:12]} vs {now[:12]}); traffic not reported")
+    return traffic, f"from {os.path.relpath(path, REPO)} (kernel sources {now[:12]})"
 
 
 def copy_bandwidth(mib: int, reps: int) -> float:
@@ -290,6 +314,7 @@ def main():
     cpu_all = None
     if not args.skip_cpu:
         import oracle
+        oracle.use_native()  # -O3 -march=native for this host, as the reference is built
         o = oracle.Oracle(desc)
         o.root_ac()
         k = min(args.cpu_sample, ntests)
@@ -299,8 +324,8 @@ def main():
         cdt = time.perf_counter() - tc
         assert (cf == fail[idx]).all(), "device and CPU oracle disagree"
         cpu = {"value": k / cdt, "unit": "fixpoints/s", "cores": 1, "kind": "port",
-               "sample": f"{k} singleton tests of the same pass (AC3bit restatement, "
-                         f"{cdt:.1f} s single-threaded)"}
+               "sample": f"{k} singleton tests of the same pass (AC3bit restatement built "
+                         f"-O3 -march=native, {cdt:.1f} s single-threaded)"}
         log(f"cpu oracle: {cpu['value']:.1f} fixpoints/s")
         # the same restatement on every thread of the host share
         th = max(1, min(args.cpu_threads, os.cpu_count() or 1))
@@ -361,8 +386,9 @@ def main():
             except Exception as exc:  # reported in the line, never hidden
                 log(f"{key} failed: {exc!r}")
                 out[key] = {"error": repr(exc)}
-    traffic = profiled_traffic(args.sac_mode)
+    traffic, note = profiled_traffic(args.sac_mode)
     out["roofline"]["traffic"] = traffic
+    out["roofline"]["traffic_note"] = note
     if traffic:
         out["measured_dram_gbps"] = traffic / (avg_kernel_ms * 1e-3) / 1e9
     m.close()

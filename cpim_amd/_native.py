@@ -123,6 +123,8 @@ class CSolveOpts(C.Structure):
         ("trace_capacity", C.c_int32),
         ("var_heuristic", C.c_int32),
         ("reserved", C.c_int32),
+        ("max_solutions", C.c_int64),
+        ("solution_capacity", C.c_int64),
     ]
 
 

@@ -472,6 +472,10 @@ struct SacScratch {
   long long cap = 0;
   DevAlloc stests, perm, gfirst;  // bit-sliced path: sorted tests, their order, group starts
   long long scap = 0, gcap = 0;
+  // removal slices of the wide bit-sliced kernel: zeroed when allocated, and
+  // left all zero by every launch of that kernel
+  DevAlloc dead;
+  long long dead_bytes = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
   SacScratch() = default;
   SacScratch(const SacScratch&) = delete;
@@ -694,14 +698,20 @@ unsigned long long fixpoint_budget_ticks(const cpim_gmodel* m) {
                 : (unsigned long long)(khz * kAsyncBudgetMs);
 }
 
-// Workgroups per CU of the persistent fixpoint (256 threads each).
-constexpr int kPersistentBlocksPerCu = 2;
+// Workgroups per CU of the persistent fixpoint (kPersistentThreads each): one
+// workgroup of 16 waves per CU is what the kernel's ~105 VGPRs allow, and it
+// keeps the barrier at one arrival per CU.
+constexpr int kPersistentBlocksPerCu = 1;
 
 // Parts per frontier word of a grid round: the smallest power of two (at
-// most 32) that gives about four work units per wave of `waves`.
-int frontier_parts(int nfw, long long waves) {
+// most 32) that gives about `per_wave` work units per wave of `waves`.  One
+// word-domain networks revise kPairBatch constraints per wave step, so one
+// unit of several constraints per wave suffices there; otherwise four units
+// of fewer constraints.
+int frontier_parts(int nfw, long long waves, int W) {
+  const long long per_wave = W == 1 ? 1 : 4;
   int parts = 1;
-  while (parts < 32 && (long long)nfw * parts < 4 * waves) parts *= 2;
+  while (parts < 32 && (long long)nfw * parts < per_wave * waves) parts *= 2;
   return parts;
 }
 
@@ -709,6 +719,54 @@ unsigned long long slot_sum(const unsigned long long (&v)[kStatSlots]) {
   unsigned long long t = 0;
   for (unsigned long long x : v) t += x;
   return t;
+}
+
+// Start of a frontier fixpoint: control block and frontiers cleared, the
+// first frontier seeded with every constraint (assigned_var < 0) or with the
+// constraints of assigned_var.
+void gac_start(cpim_gmodel* m, int assigned_var, unsigned* F, GacCtl* ctl) {
+  const long long words = 3ll * m->nfw;
+  const int blocks = (int)std::max<long long>(1, std::min<long long>((words + 255) / 256, 1024));
+  hipLaunchKernelGGL(k_gac_init, dim3(blocks), dim3(256), 0, m->stream, ctl, F, m->nfw, m->dm.num_active,
+                     assigned_var < 0 ? 1 : 0);
+  if (assigned_var >= 0)
+    hipLaunchKernelGGL(k_seed_var, dim3(1), dim3(256), 0, m->stream, F, m->dm.adj_off.as<int>(),
+                       m->dm.adj.as<int4>(), assigned_var);
+  HIPC(hipGetLastError());
+}
+
+// The persistent fixpoint kernel on the current level, after gac_start.
+// Every workgroup must be resident for the grid barrier: at most the
+// occupancy limit per CU (and at most kPersistentBlocksPerCu) on every CU.
+// If some were not resident, the barrier's bounded wait turns that into the
+// time-budget error the caller reports (ctl->err).
+void persistent_launch(cpim_gmodel* m, unsigned* F, GacCtl* ctl) {
+  const int L = m->current_level;
+  const int threads = kPersistentThreads;
+  int per_cu = 0;
+  HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gac_persistent, threads, 0));
+  per_cu = std::max(1, std::min(per_cu, kPersistentBlocksPerCu));
+  const int waves_per_block = threads / 64;
+  const int nfw = m->nfw;
+  const int parts = frontier_parts(nfw, (long long)per_cu * waves_per_block * m->props.cus, m->dm.W);
+  const long long units = (long long)nfw * parts;
+  const int grid = (int)std::max<long long>(
+      1, std::min<long long>((units + waves_per_block - 1) / waves_per_block, (long long)per_cu * m->props.cus));
+  const long long max_rounds = (long long)m->dm.n * m->dm.max_dom + 2;
+  const int mr = (int)std::min<long long>(max_rounds, INT_MAX);
+  const unsigned long long max_ticks = fixpoint_budget_ticks(m);
+  hipLaunchKernelGGL(k_gac_persistent, dim3(grid), dim3(threads), 0, m->stream, m->dm.net(), m->level_dom(L),
+                     m->level_size(L), F, nfw, ctl, mr, parts, max_ticks);
+  HIPC(hipGetLastError());
+}
+
+void fill_gac_stats(const GacCtl& res, int rounds, double kms, cpim_gac_stats* st) {
+  st->iterations = rounds;
+  st->deletions = (int32_t)slot_sum(res.deletions);
+  st->inconsistent = res.wipe_round != INT_MAX;
+  st->converged = st->inconsistent ? 0 : res.converged;
+  st->revisions = (int64_t)slot_sum(res.revisions);
+  st->kernel_ms = kms;
 }
 
 void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats* st) {
@@ -722,39 +780,11 @@ void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats*
   m->gac_events();
   double kms = 0.0;
   HIPC(hipEventRecord(m->gev[0], m->stream));
-  {
-    const long long words = 3ll * nfw;
-    const int blocks = (int)std::max<long long>(1, std::min<long long>((words + 255) / 256, 1024));
-    hipLaunchKernelGGL(k_gac_init, dim3(blocks), dim3(256), 0, m->stream, ctl, F, nfw, m->dm.num_active,
-                       assigned_var < 0 ? 1 : 0);
-  }
-  if (assigned_var >= 0)
-    hipLaunchKernelGGL(k_seed_var, dim3(1), dim3(256), 0, m->stream, F, m->dm.adj_off.as<int>(),
-                       m->dm.adj.as<int4>(), assigned_var);
-  HIPC(hipGetLastError());
-  DevNet net = m->dm.net();
-  const int threads = 256;
-  const long long max_rounds = (long long)m->dm.n * m->dm.max_dom + 2;
+  gac_start(m, assigned_var, F, ctl);
   int rounds = 0;
   GacCtl res{};
   if (persistent) {
-    // Every workgroup must be resident for the grid barrier: at most the
-    // occupancy limit per CU (and at most kPersistentBlocksPerCu) on every CU.
-    // If some were not resident, the barrier's bounded wait turns that into
-    // the time-budget error below.
-    int per_cu = 0;
-    HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gac_persistent, threads, 0));
-    per_cu = std::max(1, std::min(per_cu, kPersistentBlocksPerCu));
-    const int waves_per_block = threads / 64;
-    const int parts = frontier_parts(nfw, (long long)per_cu * waves_per_block * m->props.cus);
-    const long long units = (long long)nfw * parts;
-    const int grid = (int)std::max<long long>(
-        1, std::min<long long>((units + waves_per_block - 1) / waves_per_block, (long long)per_cu * m->props.cus));
-    const int mr = (int)std::min<long long>(max_rounds, INT_MAX);
-    const unsigned long long max_ticks = fixpoint_budget_ticks(m);
-    hipLaunchKernelGGL(k_gac_persistent, dim3(grid), dim3(threads), 0, m->stream, net, dom, size, F, nfw, ctl, mr,
-                       parts, max_ticks);
-    HIPC(hipGetLastError());
+    persistent_launch(m, F, ctl);
     HIPC(hipEventRecord(m->gev[1], m->stream));
     HIPC(hipMemcpyAsync(m->ctl_host, ctl, sizeof(res), hipMemcpyDeviceToHost, m->stream));
     m->sync();
@@ -764,7 +794,10 @@ void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats*
     rounds = res.rounds;
   } else {
     // about four work units per resident wave (32 per CU)
-    const int parts = frontier_parts(nfw, 32ll * m->props.cus);
+    DevNet net = m->dm.net();
+    const int threads = 256;
+    const long long max_rounds = (long long)m->dm.n * m->dm.max_dom + 2;
+    const int parts = frontier_parts(nfw, 32ll * m->props.cus, m->dm.W);
     const int waves_per_block = threads / 64;
     const long long units = (long long)nfw * parts;
     const int grid = (int)std::max<long long>(1, std::min<long long>((units + waves_per_block - 1) / waves_per_block,
@@ -787,12 +820,7 @@ void gac_grid(cpim_gmodel* m, int assigned_var, bool persistent, cpim_gac_stats*
       }
     }
   }
-  st->iterations = rounds;
-  st->deletions = (int32_t)slot_sum(res.deletions);
-  st->inconsistent = res.wipe_round != INT_MAX;
-  st->converged = st->inconsistent ? 0 : res.converged;
-  st->revisions = (int64_t)slot_sum(res.revisions);
-  st->kernel_ms = kms;
+  fill_gac_stats(res, rounds, kms, st);
 }
 
 void gac_async(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
@@ -924,6 +952,7 @@ int cpim_gmodel_enforce_gac(cpim_gmodel* m, int32_t assigned_var, int32_t mode, 
     gac(m, assigned_var, mode, st);
   });
 }
+
 
 int cpim_gmodel_create_new_level(cpim_gmodel* m, int32_t* new_level) {
   return guarded([&] {
@@ -1422,7 +1451,17 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
   HIPC(hipMemsetAsync(S.revs.p, 0, sizeof(unsigned long long), m->stream));
   const long long slice_bytes = 8ll * m->dm.n * m->dm.dpad * M;
   const long long per_launch = std::max<long long>(1, std::min<long long>(groups, kSliceBudgetBytes / slice_bytes));
-  u64* slices = reinterpret_cast<u64*>(m->get_scratch(slice_bytes * per_launch));
+  u64* slices = nullptr;
+  if (wide) {
+    if (slice_bytes * per_launch > S.dead_bytes) {
+      S.dead = DevAlloc((size_t)(slice_bytes * per_launch));
+      S.dead_bytes = slice_bytes * per_launch;
+      HIPC(hipMemsetAsync(S.dead.p, 0, (size_t)S.dead_bytes, m->stream));
+    }
+    slices = S.dead.as<u64>();
+  } else {
+    slices = reinterpret_cast<u64*>(m->get_scratch(slice_bytes * per_launch));
+  }
   const int L = m->current_level;
   DevNet net = m->dm.net();
   int threads = 0;
@@ -1659,12 +1698,19 @@ int cpim_gmodel_solve(cpim_gmodel* m, const cpim_solve_opts* opts, cpim_search_s
     REQUIRE(o.reserved == 0, "cpim_solve_opts.reserved must be 0");
     REQUIRE(o.trace_capacity >= 0, "negative trace capacity");
     REQUIRE(o.trace_capacity == 0 || trace, "trace buffer is null");
+    REQUIRE(o.max_solutions >= 0 && o.solution_capacity >= 0, "negative solution count");
     REQUIRE(m->current_level == 0, "the device solver starts from level 0");
     DeviceGuard g(m->device);
     std::memset(st, 0, sizeof(*st));
     const int n = m->dm.n;
     const long long t0 = now_ns();
-    DevAlloc state(sizeof(MacState)), stack(sizeof(int2) * n), sol(sizeof(int) * n);
+    const long long sol_cap = std::max<long long>(1, o.solution_capacity);
+    DevAlloc state(sizeof(MacState)), stack(sizeof(int2) * n), sol(sizeof(int) * n * sol_cap);
+    // the search keeps its depth-0 trail in level 0 and rewrites it when it
+    // backtracks at the root; level 0 gets the root closure back afterwards,
+    // as the reference leaves it (GModelSolver::Solve runs EnforceGAC on
+    // level 0 and searches on the levels above)
+    DevAlloc root(sizeof(u64) * m->level_words());
     DevAlloc tr(sizeof(NodeRec) * std::max(1, o.trace_capacity));
     MacState s0{};
     s0.status = MAC_INIT;
@@ -1674,12 +1720,14 @@ int cpim_gmodel_solve(cpim_gmodel* m, const cpim_solve_opts* opts, cpim_search_s
     A.trail = m->level_dom(0);
     A.stack = stack.as<int2>();
     A.solution = sol.as<int>();
+    A.sol_cap = sol_cap;
+    A.max_solutions = o.max_solutions;
     A.trace = o.trace_capacity > 0 ? tr.as<NodeRec>() : nullptr;
     A.trace_cap = o.trace_capacity;
     A.node_limit = o.node_limit;
     A.step_budget = 0;
     A.deadline = 0;
-    A.root_out = nullptr;
+    A.root_out = root.as<u64>();
     A.nbr = o.var_heuristic == CPIM_VARH_DOM_DEG ? m->dm.nbr.as<int>() : nullptr;
     DevAlloc wdeg_weight, wdeg_stamp, wdeg_clock;
     if (o.var_heuristic == CPIM_VARH_DOM_WDEG) {
@@ -1739,18 +1787,17 @@ int cpim_gmodel_solve(cpim_gmodel* m, const cpim_solve_opts* opts, cpim_search_s
     (void)hipEventDestroy(e1);
     st->num_positive = s.npos;
     st->num_negative = s.nneg;
-    st->num_solutions = s.status == MAC_SAT ? 1 : 0;
-    st->unsolvable = s.status == MAC_UNSAT;
+    st->num_solutions = s.nsol;
+    st->unsolvable = s.status == MAC_UNSAT && s.nsol == 0;
     if (s.status == MAC_LIMIT) st->time_out = 1;
     st->gac_iterations = s.npos + s.nneg + 1;
     st->solve_time = (now_ns() - t0) * 1e-9;
     st->gac_time = ms * 1e-3;
     if (solution) {
-      if (s.status == MAC_SAT) {
-        HIPC(hipMemcpyAsync(solution, sol.p, sizeof(int) * n, hipMemcpyDeviceToHost, m->stream));
-      } else {
-        for (int v = 0; v < n; ++v) solution[v] = -1;
-      }
+      const long long kept = std::min<long long>(s.nsol, sol_cap);
+      if (kept > 0)
+        HIPC(hipMemcpyAsync(solution, sol.p, sizeof(int) * n * kept, hipMemcpyDeviceToHost, m->stream));
+      for (long long i = kept * n; i < n * (o.solution_capacity > 0 ? sol_cap : 1); ++i) solution[i] = -1;
     }
     const int tl = std::min(s.trace_len, o.trace_capacity);
     if (tl > 0) {
@@ -1766,7 +1813,8 @@ int cpim_gmodel_solve(cpim_gmodel* m, const cpim_solve_opts* opts, cpim_search_s
       }
     }
     if (trace_len) *trace_len = s.trace_len;
-    // level 0 holds the depth-0 state; its sizes must be exact again
+    HIPC(hipMemcpyAsync(m->level_dom(0), root.p, sizeof(u64) * m->level_words(), hipMemcpyDeviceToDevice,
+                        m->stream));
     hipLaunchKernelGGL(k_sizes, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->level_dom(0), m->level_size(0), n,
                        m->dm.W);
     HIPC(hipGetLastError());

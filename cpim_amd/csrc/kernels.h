@@ -128,11 +128,88 @@ __device__ __forceinline__ void grid_process_constraint(const DevNet& net, u64* 
   }
 }
 
+// Binary constraints over one-word domains (W = 1), up to kPairBatch at a
+// time, by one wave.  Both arcs of a constraint c = (x, y) are revised from
+// the same D(x) and D(y): value a of x keeps iff row a of plane 0 meets D(y),
+// value b of y keeps iff row b of plane 1 meets D(x).  With W = 1 the rows of
+// every value are loaded whatever the domains, so the domain words (lane 2p:
+// x of constraint p, lane 2p + 1: its y) and the 2 x 64 row words of every
+// constraint of the batch are loaded together, one memory round trip for the
+// batch.  Revising an arc from a domain that the other arc shrinks in the
+// same step is a chaotic-iteration step like any other: the shrunk variable
+// is pushed, so c is revised again in the next round, and the closure is the
+// same.  The removal masks are applied with atomic AND by lanes 2p / 2p + 1;
+// for a one-word domain the returned word gives the exact removals and the
+// wipe-out (a non-empty word this AND empties), so the size update needs no
+// returned value.
+constexpr int kPairBatch = 4;
+
+template <bool kCoherent>
+__device__ __forceinline__ void grid_pairs_w1(const DevNet& net, u64* dom, int* size, const int (&cs)[kPairBatch],
+                                              int k, unsigned* B, GacCtl* ctl, int r,
+                                              unsigned long long& revs, unsigned long long& dels) {
+  const int lane = lane_id();
+  int xs[kPairBatch], ys[kPairBatch];
+  long long offs[kPairBatch];
+#pragma unroll
+  for (int p = 0; p < kPairBatch; ++p) {
+    const ConMeta* mp = net.con + (p < k ? cs[p] : cs[0]);
+    xs[p] = mp->scope[0];
+    ys[p] = mp->scope[1];
+    offs[p] = mp->off;
+  }
+  // lane 2p: D(x_p), lane 2p + 1: D(y_p)
+  int myv = -1;
+#pragma unroll
+  for (int p = 0; p < kPairBatch; ++p) {
+    if (p < k && lane == 2 * p) myv = xs[p];
+    if (p < k && lane == 2 * p + 1) myv = ys[p];
+  }
+  const u64 dl = myv >= 0 ? ld_dom<kCoherent>(&dom[myv]) : 0ull;
+  u64 r0[kPairBatch], r1[kPairBatch];
+#pragma unroll
+  for (int p = 0; p < kPairBatch; ++p) {
+    const u64* plane0 = net.sup + offs[p];
+    r0[p] = p < k ? plane0[lane] : 0ull;
+    r1[p] = p < k ? plane0[net.dpad + lane] : 0ull;
+  }
+  u64 keep_l = ~0ull;  // lane 2p: new mask of x_p, lane 2p + 1: of y_p
+#pragma unroll
+  for (int p = 0; p < kPairBatch; ++p) {
+    if (p >= k) break;
+    const u64 dx = readlane64(dl, 2 * p), dy = readlane64(dl, 2 * p + 1);
+    const u64 kx = __ballot(((dx >> lane) & 1ull) && (r0[p] & dy) != 0ull);
+    const u64 ky = __ballot(((dy >> lane) & 1ull) && (r1[p] & dx) != 0ull);
+    if (lane == 2 * p) keep_l = kx;
+    if (lane == 2 * p + 1) keep_l = ky;
+  }
+  revs += 2 * k;
+  int removed_l = 0;
+  bool wiped_l = false;
+  if (myv >= 0 && (dl & ~keep_l)) {
+    const u64 old = atomicAnd(&dom[myv], keep_l);
+    removed_l = __popcll(old & ~keep_l);
+    wiped_l = old != 0ull && (old & keep_l) == 0ull;
+    if (removed_l) atomicSub(&size[myv], removed_l);
+  }
+  if (__ballot(wiped_l) && lane == 0) atomicMin(&ctl->wipe_round, r);
+  u64 changed = __ballot(removed_l > 0);
+  const int total = (int)wave_sum64_dpp((u64)removed_l);
+  dels += total;
+  while (changed) {
+    const int l = (int)__builtin_ctzll(changed);
+    changed &= changed - 1ull;
+    push_var(net, __builtin_amdgcn_readlane(myv, l), B, ctl, r);
+  }
+}
+
 // Work units are (frontier word, part) pairs: part p of word w covers the
 // constraint bits [p * 32 / parts, (p + 1) * 32 / parts) of A[w], so a round
 // over few words still spreads over many waves; waves take units in a fixed
 // stride.  kCoherent: domains and frontier words are read with agent-scope
 // loads (rounds of one launch, written by other XCDs since the last barrier).
+// With one-word domains, binary constraints go kPairBatch at a time through
+// grid_pairs_w1.
 template <bool kCoherent = false>
 __device__ __forceinline__ void grid_round(const DevNet& net, u64* dom, int* size, unsigned* F,
                                            int nfw, GacCtl* ctl, int r, int parts) {
@@ -159,7 +236,28 @@ __device__ __forceinline__ void grid_round(const DevNet& net, u64* dom, int* siz
     const int part = ui - wi * parts;
     const unsigned pmask = pbits == 32 ? ~0u : (((1u << pbits) - 1u) << (part * pbits));
     const unsigned aw = kCoherent ? (unsigned)ld_agent((const int*)&A[wi]) : A[wi];
-    unsigned bits = aw & pmask;
+    unsigned bits = (unsigned)uniform_int((int)(aw & pmask));
+    if (net.W == 1) {
+      while (bits) {
+        int cs[kPairBatch] = {0, 0, 0, 0};
+        int k = 0;
+        while (bits && k < kPairBatch) {
+          const int b = __ffs(bits) - 1;
+          bits &= bits - 1u;
+          const int c = wi * 32 + b;
+          if (net.con[c].kind != 0) {
+            grid_process_constraint<kCoherent>(net, dom, size, c, B, ctl, r, revs, dels);
+            continue;
+          }
+#pragma unroll
+          for (int p = 0; p < kPairBatch; ++p)
+            if (p == k) cs[p] = c;
+          ++k;
+        }
+        if (k) grid_pairs_w1<kCoherent>(net, dom, size, cs, k, B, ctl, r, revs, dels);
+      }
+      continue;
+    }
     while (bits) {
       const int b = __ffs(bits) - 1;
       bits &= bits - 1u;
@@ -199,12 +297,26 @@ __global__ void k_gac_round(DevNet net, u64* dom, int* size, unsigned* F, int nf
 // ===========================================================================
 enum : int { kRoundNext = 0, kRoundWipe = 1, kRoundFixpoint = 2, kRoundError = 3 };
 
+// Every write a round makes that another workgroup reads in a later round
+// (domain words, sizes, frontier bits, epochs, the cleared frontier) is an
+// agent-scope atomic, and every such read is an agent-scope atomic load.  So
+// before the barrier each wave only waits until its own memory operations
+// have completed (s_waitcnt vmcnt(0): atomics with and without return, and
+// stores), and one thread per workgroup fences: a full release fence in every
+// wave (an L2 write-back each) made a barrier cost 37 us at 1,024 waves
+// against 12 us for a launch and sync.
+__device__ __forceinline__ void wait_memory_ops() {
+  // vmcnt(0); expcnt and lgkmcnt left at their maxima (no wait)
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+}
+
 __device__ __forceinline__ int grid_barrier_decide(GacCtl* ctl, unsigned nblocks, int r,
                                                    unsigned long long t0, unsigned long long max_ticks,
                                                    int* s_decision) {
-  __threadfence();  // release: this wave's removals, pushes and epochs
+  wait_memory_ops();
   __syncthreads();
   if (threadIdx.x == 0) {
+    __threadfence();  // release: the workgroup's completed round
     const unsigned gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned arrived = atomicAdd(&ctl->bar_count, 1u) + 1u;
     int st = kRoundNext;
@@ -240,7 +352,9 @@ __device__ __forceinline__ int grid_barrier_decide(GacCtl* ctl, unsigned nblocks
   return uniform_int(*s_decision);
 }
 
-__global__ void __launch_bounds__(256) k_gac_persistent(DevNet net, u64* dom, int* size, unsigned* F, int nfw,
+constexpr int kPersistentThreads = 1024;
+
+__global__ void __launch_bounds__(kPersistentThreads) k_gac_persistent(DevNet net, u64* dom, int* size, unsigned* F, int nfw,
                                                         GacCtl* ctl, int max_rounds, int parts,
                                                         unsigned long long max_ticks) {
   __shared__ int s_decision;
@@ -1190,14 +1304,26 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
 }
 
 // ---------------------------------------------------------------------------
-// Wide bit-sliced singleton tests: 64 M tests per workgroup (M = 1, 2, 4),
-// sl[(v * dpad + b) * M + m] holding tests 64m..64m+63.  Within a round the
-// workgroup walks the changed variables t one at a time: all threads stage
-// t's masks into a shared, double-buffered LDS area (one barrier per t),
-// then the waves take t's arcs u <- t from a shared counter.  One support
-// entry serves 64 M tests; the revision is the one of k_sliced_tests applied
-// to M masks at once (same fixpoint argument, same exact end-of-round
-// wipe-out check).
+// Wide bit-sliced singleton tests: 64 M tests per workgroup (M = 1, 2, 4).
+// Within a round the workgroup walks the changed variables t one at a time:
+// all threads stage t's masks into a shared, double-buffered LDS area (one
+// barrier per t), then the waves take t's arcs u <- t from a shared counter.
+// One support entry serves 64 M tests; the revision is the one of
+// k_sliced_tests applied to M masks at once (same fixpoint argument, same
+// exact end-of-round wipe-out check).
+//
+// The slices hold changes from the base level, not survivors: bit k of
+// sl[(v * dpad + b) * M + m] flips the base state of value b of v in test
+// 64 m + k.  For a base value a set bit means the test removed it; for a
+// value outside the base domain a set bit means the test holds it, which
+// only the test's own value can be (a test (x, a) reduces x to {a} whether
+// or not a is in the base domain).  So value b is alive in test k iff the
+// bit differs from "b in the base domain".  The region of a workgroup is all
+// zero when the kernel starts (every test holds exactly the base domain), a
+// variable's slice is written only when some test changes one of its values,
+// and the workgroup zeroes the slices of the variables it wrote (its touched
+// set) before it ends, so the buffer is all zero again for the next launch.
+// The caller zeroes the buffer once, when it allocates it.
 // ---------------------------------------------------------------------------
 template <int M>
 struct WideCtl {
@@ -1269,6 +1395,18 @@ __device__ __forceinline__ void clear_supported_wide(u64 (&pend)[M], unsigned bi
 constexpr int kNibStep = 4;
 constexpr unsigned kNibStepMask = kNibStep >= 8 ? 0xffffffffu : ((1u << (4 * kNibStep)) - 1u);
 
+// Word of mask mm of subset `sub` in the table of nibble q.  With four masks
+// an entry (32 bytes) is split in two 16-byte halves kept in separate
+// 256-byte planes of the nibble's block: plane h holds masks 2h, 2h + 1 of
+// the 16 subsets.  The 16 entries a wave's lanes pick from in one read then
+// sit on distinct banks (one 16-byte slot each) instead of colliding every
+// fourth entry.
+template <int M>
+__device__ __forceinline__ int nib_word(int q, int sub, int mm) {
+  if constexpr (M == 4) return q * 64 + (mm >> 1) * 32 + sub * 2 + (mm & 1);
+  else return (q * 16 + sub) * M + mm;
+}
+
 template <int M>
 __device__ __forceinline__ void clear_supported_nib(u64 (&pend)[M], unsigned bits, const u64* tab) {
   for (int q = 0; q < 8 && bits && any_mask<M>(pend); q += kNibStep) {
@@ -1278,9 +1416,9 @@ __device__ __forceinline__ void clear_supported_nib(u64 (&pend)[M], unsigned bit
 #pragma unroll
     for (int i = 0; i < kNibStep; ++i) {
       const unsigned s = (bits >> (4 * (q + i))) & 15u;
-      const u64* e = tab + (long long)(((q + i) << 4) | s) * M;
+      const u64* e = tab + nib_word<M>(q + i, (int)s, 0);
 #pragma unroll
-      for (int m = 0; m < M; ++m) acc[m] |= e[m];
+      for (int m = 0; m < M; ++m) acc[m] |= e[nib_word<M>(0, 0, m)];
     }
     bits &= ~(kNibStepMask << (4 * q));
 #pragma unroll
@@ -1297,12 +1435,14 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
   bool changed = false;
   for (int j = 0; j < W; ++j) {
     const int b = j * 64 + lane;
+    const bool in = (base_u[j] >> lane) & 1ull;
+    const u64 flip = in ? ~0ull : 0ull;  // a set slice bit flips the base state
     u64 pend[M];
     if (touched_u) {
 #pragma unroll
-      for (int m = 0; m < M; ++m) pend[m] = ld_dom<true>(&slu[(long long)b * M + m]) & act[m];
+      for (int m = 0; m < M; ++m)
+        pend[m] = all[m] & act[m] & (ld_dom<true>(&slu[(long long)b * M + m]) ^ flip);
     } else {
-      const bool in = (base_u[j] >> lane) & 1ull;
 #pragma unroll
       for (int m = 0; m < M; ++m) pend[m] = in ? (all[m] & act[m]) : 0ull;
     }
@@ -1333,8 +1473,15 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
 #pragma unroll
       for (int m = 0; m < M; ++m) {
         if (pend[m]) {
-          const u64 old = atomicAnd(&slu[(long long)b * M + m], ~pend[m]);
-          if (old & pend[m]) changed = true;
+          // removal: set the bit of a base value, clear that of a value a
+          // test made present
+          if (in) {
+            const u64 old = atomicOr(&slu[(long long)b * M + m], pend[m]);
+            if (pend[m] & ~old) changed = true;
+          } else {
+            const u64 old = atomicAnd(&slu[(long long)b * M + m], ~pend[m]);
+            if (pend[m] & old) changed = true;
+          }
         }
       }
     }
@@ -1385,13 +1532,6 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
   const int nbuf = 2;  // double buffer
   u64* atw0 = stage0 + nbuf * stage_words;
 
-  // slices of the base level: every test holds every base value
-  for (int v = wave; v < n; v += nwaves)
-    for (int b = lane; b < dpad; b += 64) {
-      const bool in = (base_dom[(long long)v * W + (b >> 6)] >> (b & 63)) & 1ull;
-#pragma unroll
-      for (int m = 0; m < M; ++m) sl[((long long)v * dpad + b) * M + m] = in ? all[m] : 0ull;
-    }
   for (int i = tid; i < nvw; i += nthreads) {
     cur[i] = 0u;
     nxt[i] = 0u;
@@ -1410,15 +1550,15 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
   }
   __threadfence();
   __syncthreads();
-  // test k reduces its variable x_k to {a_k}
+  // test k reduces its variable x_k to {a_k}: flip every base value but a_k,
+  // and a_k if it is not a base value
   for (int k = wave; k < tc; k += nwaves) {
     const int2 tk = tests[tf + k];
     const int m = k >> 6;
     const u64 bit = 1ull << (k & 63);
     for (int b = lane; b < dpad; b += 64) {
-      u64* p = &sl[((long long)tk.x * dpad + b) * M + m];
-      if (b == tk.y) atomicOr(p, bit);
-      else atomicAnd(p, ~bit);
+      const bool in = (base_dom[(long long)tk.x * W + (b >> 6)] >> (b & 63)) & 1ull;
+      if (in != (b == tk.y)) atomicOr(&sl[((long long)tk.x * dpad + b) * M + m], bit);
     }
   }
   if (tid < tc) {
@@ -1461,6 +1601,14 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
       u64* atw = atw0 + (i & 1) * W;
       // stage t: thread v holds value v's M masks; one wave covers one word
       const u64* slt = sl + (long long)t * dpad * M;
+      const u64* base_t = base_dom + (long long)t * W;
+      // masks of value v of t: the live tests that still hold it
+      auto alive_of = [&](int v, int mm) -> u64 {
+        const u64 flip = ((base_t[v >> 6] >> (v & 63)) & 1ull) ? ~0ull : 0ull;
+        const int kk = tc - 64 * mm;
+        const u64 all_m = kk >= 64 ? ~0ull : (kk > 0 ? ((1ull << kk) - 1ull) : 0ull);
+        return all_m & (ld_dom<true>(&slt[(long long)v * M + mm]) ^ flip);
+      };
       // every wave finished the arcs of buffer (i & 1)'s previous variable
       // before the last barrier
       if (tid == 0) ctl.arc_ctr[i & 1] = 0;
@@ -1470,15 +1618,14 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
           const int q = it / M, mm = it - q * M;
           u64 x[4];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) x[i] = ld_dom<true>(&slt[(long long)(4 * q + i) * M + mm]);
-          u64* e = stage + (long long)q * 16 * M + mm;
+          for (int i = 0; i < 4; ++i) x[i] = alive_of(4 * q + i, mm);
 #pragma unroll
           for (int sub = 0; sub < 16; ++sub) {
             u64 o = 0ull;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
               if (sub & (1 << i)) o |= x[i];
-            e[(long long)sub * M] = o;
+            stage[nib_word<M>(q, sub, mm)] = o;
           }
         }
       }
@@ -1487,7 +1634,7 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
         u64* dst = stage + (long long)stage_pos(v) * M;
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-          const u64 x = ld_dom<true>(&slt[(long long)v * M + m]);
+          const u64 x = alive_of(v, m);
           if (!kNib) dst[m] = x;
           alive |= x & act[m];
         }
@@ -1529,7 +1676,10 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
 #pragma unroll
       for (int m = 0; m < M; ++m) {
         u64 o = 0ull;
-        for (int b = lane; b < dpad; b += 64) o |= ld_dom<true>(&sl[((long long)v * dpad + b) * M + m]);
+        for (int b = lane; b < dpad; b += 64) {
+          const u64 flip = ((base_dom[(long long)v * W + (b >> 6)] >> (b & 63)) & 1ull) ? ~0ull : 0ull;
+          o |= all[m] & (ld_dom<true>(&sl[((long long)v * dpad + b) * M + m]) ^ flip);
+        }
         o = wave_or64_dpp(o);
         alive_all[m] &= o;
       }
@@ -1554,6 +1704,12 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
   if (tid < tc)
     fail[perm ? perm[tf + tid] : tf + tid] = ((ctl.act[tid >> 6] >> (tid & 63)) & 1ull) ? 0 : 1;
   if (tid == 0 && ctl.revs) atomicAdd(revs_out, ctl.revs);
+  // leave the region all zero: clear the slices of every variable written
+  for (int v = wave; v < n; v += nwaves) {
+    if (!uniform_int((int)bit_of(touched, v))) continue;
+    u64* p = sl + (long long)v * dpad * M;
+    for (int i = lane; i < dpad * M; i += 64) p[i] = 0ull;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1566,13 +1722,14 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
 // `step_budget` nodes and resumed by a later launch (serving).
 // ---------------------------------------------------------------------------
 struct MacState {
-  int status;  // 0 init, 1 running, 2 solution, 3 unsat, 4 limit reached
+  int status;  // 0 init, 1 running, 2 solution, 3 unsat or enumeration done, 4 limit reached
   int depth;
   long long npos;
   long long nneg;
   long long revisions;
   int trace_len;
   int steps;
+  long long nsol;  // solutions found so far
 };
 
 struct NodeRec {
@@ -1617,7 +1774,11 @@ struct MacArgs {
   MacState* state;
   u64* trail;          // [n + 1][n][W]; trail[0] holds the start domains
   int2* stack;         // [n]
-  int* solution;       // [n] or NULL
+  int* solution;       // [sol_cap][n] or NULL: solution k at solution + k n
+  long long sol_cap;   // solutions the buffer holds (0 counts as 1)
+  // 0 or 1: stop at the first solution; k > 1: count every solution, treat it
+  // as a failed node and go on, until the tree is exhausted or k are found
+  long long max_solutions;
   NodeRec* trace;      // [trace_cap] or NULL
   int trace_cap;
   long long node_limit;   // total (0 = none)
@@ -1838,6 +1999,7 @@ __device__ __forceinline__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* 
   int depth = A.state->depth;
   long long npos = A.state->npos;
   long long nneg = A.state->nneg;
+  long long nsol = A.state->nsol;
   long long nodes_here = 0;
   if (threadIdx.x == 0) {
     ctl->trace_len = A.state->trace_len;
@@ -1847,6 +2009,7 @@ __device__ __forceinline__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* 
   depth = uniform_int(depth);
   npos = uniform_ll(npos);
   nneg = uniform_ll(nneg);
+  nsol = uniform_ll(nsol);
   __syncthreads();
   if (status == MAC_INIT) {
     wg_copy_words(mem.dom, A.trail, nw);
@@ -1934,18 +2097,23 @@ __device__ __forceinline__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* 
     bool ok = A.weight ? wg_ac3(net, mem, ctl, A, x) : wg_fixpoint<4>(net, mem, ctl, false);
     wg_trace(net, mem, ctl, A, 1, x, a, ok);
     if (ok && depth == n) {
-      if (A.solution)
+      if (A.solution && nsol < (A.sol_cap > 0 ? A.sol_cap : 1))
         for (int v = threadIdx.x; v < n; v += blockDim.x) {
           int sv = -1;
           for (int w = 0; w < W && sv < 0; ++w) {
             const u64 dw = mem.dom[(long long)v * W + w];
             if (dw) sv = w * 64 + (int)__builtin_ctzll(dw);
           }
-          A.solution[v] = sv;
+          A.solution[nsol * n + v] = sv;
         }
-      status = MAC_SAT;
+      ++nsol;
+      if (A.max_solutions <= 1 || nsol >= A.max_solutions) {
+        status = MAC_SAT;
+        __syncthreads();
+        break;
+      }
+      ok = false;  // enumeration: go on as after a failed node
       __syncthreads();
-      break;
     }
     while (!ok && depth > 0) {
       --depth;
@@ -1985,6 +2153,7 @@ __device__ __forceinline__ void wg_mac_run(const DevNet& net, WgMem mem, WgCtl* 
     A.state->depth = depth;
     A.state->npos = npos;
     A.state->nneg = nneg;
+    A.state->nsol = nsol;
     A.state->revisions += (long long)ctl->revs;
     A.state->trace_len = ctl->trace_len;
     A.state->steps += 1;

@@ -157,14 +157,23 @@ class GModel:
         return st
 
     def EnforceGAC_Persistent(self, verbose: bool = True, assigned_var: int = -1) -> GacStats:
-        """All rounds in one launch.  Served by the asynchronous one-launch
-        kernel (no grid barrier); the cooperative grid-barrier kernel is
-        switched off in the engine."""
-        st = self._gac(assigned_var, N.CPIM_GAC_ASYNC)
+        """All rounds in one launch of co-resident workgroups, a grid barrier
+        between rounds (src/GModel.cu:1222-1391)."""
+        st = self._gac(assigned_var, N.CPIM_GAC_PERSISTENT)
         if verbose:
             print(f"[GAC-Persistent] iterations={st.iterations} deletions={st.deletions} "
                   f"inconsistent={'true' if st.inconsistent else 'false'} "
                   f"converged={'true' if st.converged else 'false'}")
+        return st
+
+    def EnforceGAC_Legacy(self, verbose: bool = True) -> GacStats:
+        """Deprecated in the reference (include/GModel.cuh:199-201: the
+        event-list fixpoint of CsCheckMainKernel, one launch per round from
+        every constraint).  Its closure is the host-driven rounds' one."""
+        st = self._gac(-1, N.CPIM_GAC_PLAIN)
+        if verbose:
+            print(f"[GAC-Legacy] iterations={st.iterations} deletions={st.deletions} "
+                  f"inconsistent={'true' if st.inconsistent else 'false'}")
         return st
 
     def EnforceGAC_Async(self, assigned_var: int = -1) -> GacStats:
@@ -280,14 +289,24 @@ class GModel:
         (domain size over neighbour count, MAC.cpp:371-384) or "dom/wdeg"
         (domain size over failure-weighted degree, MAC.cpp:386-411; its
         fixpoints run in AC3's order on one wave)."""
+        stats, sols, trace = self.SolveAll(1, 1, time_limit_ms, node_limit, trace_capacity, heuristic)
+        return stats, (sols[0] if len(sols) else None), trace
+
+    def SolveAll(self, max_solutions: int, solution_capacity: int, time_limit_ms: int = 0,
+                 node_limit: int = 0, trace_capacity: int = 0, heuristic: str = "dom"):
+        """Device MAC that counts up to max_solutions solutions (each found
+        solution is then treated as a failed node) and keeps the first
+        solution_capacity of them.  Returns (stats, solutions [k, n], trace)."""
         o = N.CSolveOpts()
         o.time_limit_ms = time_limit_ms
         o.node_limit = node_limit
         o.branching = N.CPIM_BRANCH_BINARY
         o.trace_capacity = trace_capacity
         o.var_heuristic = _var_heuristic(heuristic)
+        o.max_solutions = max_solutions
+        o.solution_capacity = max(1, solution_capacity)
         st = N.CSearchStats()
-        sol = np.full(self.num_vars, -1, np.int32)
+        sol = np.full(max(1, solution_capacity) * self.num_vars, -1, np.int32)
         tr = (N.CNodeRecord * max(1, trace_capacity))()
         tl = C.c_int32(0)
         N.check(N.gpu_lib().cpim_gmodel_solve(self.handle, C.byref(o), C.byref(st),
@@ -297,7 +316,8 @@ class GModel:
                                     bool(st.unsolvable), st.solve_time, st.gac_time)
         trace = [(r.positive, r.var, r.value, r.consistent, r.dom_hash)
                  for r in tr[:min(tl.value, trace_capacity)]]
-        return stats, (sol if st.num_solutions else None), trace
+        k = min(int(st.num_solutions), max(1, solution_capacity))
+        return stats, sol[:k * self.num_vars].reshape(k, self.num_vars), trace
 
 
 class GModelAdapter:
@@ -351,14 +371,24 @@ class GModelSolver:
 
     def Solve(self, time_limit: int = 0, node_limit: int = 0,
               trace_capacity: int = 0) -> GpuSearchStatistics:
+        """With SetFindAllSolutions(True) the binary search enumerates, up to
+        SetMaxSolutions (1,000,000 by default then), as the reference's search
+        stops (src/GModelSolver.cu:125-128): every solution is counted, then
+        treated as a failed node, and GetAllSolutions() returns the first
+        ones, as many as fit in SOLUTION_BYTES of device memory
+        (num_solutions counts them all)."""
         self.solutions_ = []
         if self.branching == "binary":
-            st, sol, self.trace = self.model_.Solve(time_limit, node_limit, trace_capacity,
-                                                    self.heuristic)
-            if sol is not None:
-                self.solutions_.append([int(x) for x in sol])
+            k = max(1, self.max_solutions_) if self.find_all_solutions_ else 1
+            cap = max(1, min(k, self.SOLUTION_BYTES // (4 * max(1, self.model_.num_vars))))
+            st, sols, self.trace = self.model_.SolveAll(k, cap, time_limit, node_limit, trace_capacity,
+                                                        self.heuristic)
+            self.solutions_ = [[int(x) for x in row] for row in sols]
             return st
         return self._solve_dway(time_limit)
+
+    # device memory for the solutions kept by an enumerating search
+    SOLUTION_BYTES = 256 << 20
 
     # d-way search of the reference GModelSolver, node for node.
     def _solve_dway(self, time_limit: int) -> GpuSearchStatistics:

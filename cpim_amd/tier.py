@@ -51,8 +51,9 @@ class TierResult:
 
 
 def expand_paths(paths: Sequence[str]) -> List[str]:
-    """Files, directories (recursive *.xml) and @list files, in order, without
-    repeats."""
+    """Files, directories (recursive *.xml), @list files and bench manifests
+    (BMPath.xml: every file of every entry, src/model/libxml2_parser.cpp:165-283),
+    in order, without repeats."""
     out: List[str] = []
     seen = set()
 
@@ -77,9 +78,23 @@ def expand_paths(paths: Sequence[str]) -> List[str]:
                 found += [os.path.join(root, f) for f in files if f.endswith(".xml")]
             for f in sorted(found):
                 add(f)
+        elif _is_manifest_file(p):
+            for e in xcsp.load_bench_manifest(p):
+                for f in e.files:
+                    add(f.path)
         else:
             add(p)
     return out
+
+
+def _is_manifest_file(path: str) -> bool:
+    if not path.endswith(".xml") or not os.path.isfile(path):
+        return False
+    try:
+        import xml.etree.ElementTree as ET
+        return xcsp.is_manifest(ET.parse(path).getroot())
+    except Exception:
+        return False
 
 
 def parse_solver_output(output: str, returncode: Optional[int]) -> dict:

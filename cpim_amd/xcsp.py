@@ -20,6 +20,7 @@ or skipped and listed when skip_unsupported is set.
 """
 from __future__ import annotations
 
+import os
 import re
 import xml.etree.ElementTree as ET
 from dataclasses import dataclass, field
@@ -106,13 +107,128 @@ class XcspInstance:
         return [self.var_values[v][int(a)] for v, a in enumerate(canonical)]
 
 
+# ---------------------------------------------------------------------------
+# Bench manifests (BMPath.xml): <BenchMark><BMFiles><BMFile [type=...]>path
+# </BMFile>...</BMFiles></BenchMark>, src/model/libxml2_parser.cpp:165-383.
+# ---------------------------------------------------------------------------
+@dataclass
+class BenchFile:
+    path: str
+    format: str  # the presentation's format attribute ("" when absent)
+
+
+@dataclass
+class BenchEntry:
+    original_path: str
+    resolved_path: str
+    kind: str  # "file" or "dir"
+    files: List[BenchFile]
+
+
+def detect_format(path: str) -> str:
+    """The <presentation format="..."> attribute of an XCSP file, "" if none
+    (DetectFormatVersion, :385-410)."""
+    try:
+        root = ET.parse(path).getroot()
+    except ET.ParseError:
+        raise XcspError(f"Failed to parse XML file: {path}") from None
+    pres = root.find("presentation")
+    return pres.get("format", "") if pres is not None else ""
+
+
+def describe_bench_path(original: str, resolved: str) -> BenchEntry:
+    """A manifest entry resolved to a file or to every *.xml below a directory
+    (BuildBenchEntry / CollectDirectoryFiles, :284-383).  Directory files are
+    listed in sorted order (the reference walks them in directory order)."""
+    resolved = os.path.realpath(os.path.abspath(resolved))
+    if not os.path.exists(resolved):
+        raise FileNotFoundError(f"Bench path not found: {resolved}")
+    if os.path.isdir(resolved):
+        found = []
+        for root, _, files in os.walk(resolved):
+            found += [os.path.join(root, f) for f in files if f.endswith(".xml")]
+        if not found:
+            raise FileNotFoundError(f"No XML benchmarks found under {resolved}")
+        return BenchEntry(original, resolved, "dir",
+                          [BenchFile(os.path.abspath(f), detect_format(f)) for f in sorted(found)])
+    if os.path.isfile(resolved):
+        return BenchEntry(original, resolved, "file", [BenchFile(resolved, detect_format(resolved))])
+    raise XcspError(f"Bench path is neither file nor directory: {resolved}")
+
+
+def is_manifest(root: ET.Element) -> bool:
+    """A bench manifest: a <BMFiles> section or <BMFile> entries under the root.
+    (The reference's Parse looks for a <BMFile> child of the root, :88-90,
+    while its own BMPath.xml and LoadBenchManifest keep the entries in a
+    <BMFiles> section; both are accepted here.)"""
+    return root.find("BMFiles") is not None or root.find("BMFile") is not None
+
+
+def load_bench_manifest(path: str) -> List[BenchEntry]:
+    """Entries of a manifest (LoadBenchManifest, :165-282).  A relative manifest
+    path is tried against the working directory, then its parent; an entry
+    against the manifest's directory, then (relative entries) the working
+    directory and the manifest's grandparent directory; the first that exists
+    is used.  type="dir" / "directory" / "folder" must resolve to a directory,
+    any other type to a file."""
+    cands = [os.path.abspath(path)]
+    if not os.path.isabs(path):
+        cands.append(os.path.abspath(os.path.join(os.path.dirname(os.getcwd()), path)))
+    manifest = next((c for c in cands if os.path.exists(c)), None)
+    if manifest is None:
+        raise FileNotFoundError(f"Bench manifest not found: {cands[0]}")
+    try:
+        root = ET.parse(manifest).getroot()
+    except ET.ParseError:
+        raise XcspError(f"Failed to parse XML file: {manifest}") from None
+    section = root.find("BMFiles")
+    if section is None:
+        raise XcspError("Bench manifest missing <BMFiles> section")
+    mdir = os.path.dirname(manifest)
+    entries: List[BenchEntry] = []
+    for node in section.findall("BMFile"):
+        raw = (node.text or "").strip()
+        if not raw:
+            continue
+        tries = [os.path.join(mdir, raw)]
+        if not os.path.isabs(raw):
+            tries += [os.path.abspath(raw), os.path.abspath(os.path.join(os.path.dirname(mdir), raw))]
+        entry = None
+        for t in tries:
+            try:
+                entry = describe_bench_path(raw, t)
+                break
+            except FileNotFoundError:
+                continue
+        if entry is None:
+            raise FileNotFoundError(f"Bench path not found: {os.path.realpath(tries[0])}")
+        kind = node.get("type")
+        if kind is not None:
+            want = "dir" if kind.lower() in ("dir", "directory", "folder") else "file"
+            if entry.kind != want:
+                raise XcspError(f"Manifest entry {raw} expected {'directory' if want == 'dir' else 'file'} "
+                                f"but resolved to {'directory' if entry.kind == 'dir' else 'file'}")
+        entries.append(entry)
+    if not entries:
+        raise XcspError("Bench manifest did not contain usable BMFile entries")
+    return entries
+
+
 def read(source: Union[str, bytes], max_expanded_tuples: int = 1_000_000,
          skip_unsupported: bool = False) -> XcspInstance:
-    """Read an XCSP 2.x instance from a path or from XML text."""
+    """Read an XCSP 2.x instance from a path or from XML text.  A bench
+    manifest path reads the first file its entries resolve to, as the
+    reference's parser does (:88-116)."""
     if isinstance(source, bytes) or (isinstance(source, str) and source.lstrip().startswith("<")):
         root = ET.fromstring(source)
     else:
         root = ET.parse(source).getroot()
+        if is_manifest(root):
+            entries = load_bench_manifest(source)
+            first = next((e for e in entries if e.files), None)
+            if first is None:
+                raise XcspError("Manifest did not resolve to any XML files")
+            return read(first.files[0].path, max_expanded_tuples, skip_unsupported)
     pres = root.find("presentation")
     name = pres.get("name", "") if pres is not None else ""
     domains: Dict[str, np.ndarray] = {}

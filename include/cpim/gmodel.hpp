@@ -20,6 +20,7 @@
 // throws on inconsistency (it reports it in GacStats), like the reference.
 #pragma once
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <stdexcept>
@@ -84,10 +85,16 @@ class GModel {
   GacStats EnforceGAC(bool verbose = true, int assigned_var = -1) {
     return run(assigned_var, CPIM_GAC_PLAIN, verbose, "[GAC-Bitmap]");
   }
-  // All rounds in one launch: served by the asynchronous kernel (the
-  // cooperative grid-barrier kernel is switched off in the engine).
+  // All rounds in one launch of co-resident workgroups, a grid barrier
+  // between rounds.
   GacStats EnforceGAC_Persistent(bool verbose = true, int assigned_var = -1) {
-    return run(assigned_var, CPIM_GAC_ASYNC, verbose, "[GAC-Persistent]");
+    return run(assigned_var, CPIM_GAC_PERSISTENT, verbose, "[GAC-Persistent]");
+  }
+  // Deprecated in the reference (include/GModel.cuh:199-201): the event-list
+  // fixpoint from every constraint, one launch per round; served by the
+  // host-driven rounds, whose closure is the same.
+  [[deprecated("use EnforceGAC")]] GacStats EnforceGAC_Legacy(bool verbose = true) {
+    return run(-1, CPIM_GAC_PLAIN, verbose, "[GAC-Legacy]");
   }
   // Extensions: one-workgroup LDS fixpoint, and the autotuned choice.
   GacStats EnforceGAC_Block(int assigned_var = -1) { return run(assigned_var, CPIM_GAC_BLOCK, false, ""); }
@@ -232,6 +239,14 @@ class GModelSolver {
     var_heuristic_ = h;
   }
 
+  // With SetFindAllSolutions(true) the binary search enumerates, up to
+  // SetMaxSolutions (1,000,000 by default then), as the reference's search
+  // stops (src/GModelSolver.cu:125-128): each solution is counted, then
+  // treated as a failed node.  GetAllSolutions() returns the first ones, as
+  // many as fit in kSolutionBytes of device memory; num_solutions counts
+  // them all.
+  static constexpr long long kSolutionBytes = 256ll << 20;
+
   GpuSearchStatistics Solve(int time_limit = 0) {
     solutions_.clear();
     if (branching_ == Branching::kDway) return SolveDway(time_limit);
@@ -239,8 +254,13 @@ class GModelSolver {
     o.time_limit_ms = time_limit;
     o.branching = CPIM_BRANCH_BINARY;
     o.var_heuristic = var_heuristic_;
+    const int n = model_->num_vars();
+    const long long k = find_all_ ? std::max(1, max_solutions_) : 1;
+    const long long cap = std::max(1ll, std::min(k, kSolutionBytes / (4ll * std::max(1, n))));
+    o.max_solutions = k;
+    o.solution_capacity = cap;
     cpim_search_stats st{};
-    std::vector<int32_t> sol(model_->num_vars(), -1);
+    std::vector<int32_t> sol((size_t)(n * cap), -1);
     int32_t tl = 0;
     check(cpim_gmodel_solve(model_->handle(), &o, &st, sol.data(), nullptr, &tl));
     GpuSearchStatistics g;
@@ -252,7 +272,8 @@ class GModelSolver {
     g.unsolvable = st.unsolvable != 0;
     g.solve_time = st.solve_time;
     g.gac_time = st.gac_time;
-    if (st.num_solutions) solutions_.emplace_back(sol.begin(), sol.end());
+    const long long kept = std::min<long long>(st.num_solutions, cap);
+    for (long long i = 0; i < kept; ++i) solutions_.emplace_back(sol.begin() + i * n, sol.begin() + (i + 1) * n);
     return g;
   }
 

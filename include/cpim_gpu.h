@@ -275,6 +275,14 @@ typedef struct cpim_solve_opts {
   int32_t trace_capacity;/* records kept in trace (0 = none) */
   int32_t var_heuristic; /* CPIM_VARH_DOM_MIN (0), _DOM_DEG or _DOM_WDEG */
   int32_t reserved;      /* 0 */
+  /* GModelSolver::SetMaxSolutions / SetFindAllSolutions (include/GModelSolver.h:48-54):
+   * 0 or 1 = stop at the first solution; k > 1 = every solution found is
+   * counted and stored, then treated as a failed node, until the search tree
+   * is exhausted or k solutions are found (find-all: a large k) */
+  int64_t max_solutions;
+  /* solutions the `solution` buffer of cpim_gmodel_solve holds, num_vars ints
+   * each, in the order found (0 = 1); stats->num_solutions counts them all */
+  int64_t solution_capacity;
 } cpim_solve_opts;
 
 /* GpuSearchStatistics (include/GModelSolver.h:11-21) + SearchStatistics

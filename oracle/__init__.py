@@ -50,15 +50,41 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
+NATIVE_LIB_PATH = os.path.join(_HERE, "liboracle_native.so")
+
+
+def build_native() -> str:
+    """Compile liboracle_native.so with -O3 -march=native for the host it runs
+    on: the timed CPU baseline, compiled for that host's CPU.  The portable
+    liboracle.so stays the checker."""
+    src = os.path.join(_HERE, "oracle.cpp")
+    cmd = ["g++", "-O3", "-march=native", "-std=c++17", "-fPIC", "-shared", "-Wall",
+           "-o", NATIVE_LIB_PATH + ".tmp", src, "-lpthread"]
+    subprocess.run(cmd, check=True)
+    os.replace(NATIVE_LIB_PATH + ".tmp", NATIVE_LIB_PATH)
+    return NATIVE_LIB_PATH
+
+
+def use_native() -> None:
+    """Route every later Oracle through the -march=native build (bench's CPU
+    baseline leg only); must be called before the first oracle call."""
+    global _lib_path, _lib
+    if _lib is not None and _lib_path != NATIVE_LIB_PATH:
+        raise RuntimeError("the oracle library is already loaded")
+    build_native()
+    _lib_path = NATIVE_LIB_PATH
+
+
 _lib = None
+_lib_path = LIB_PATH
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if _lib_path == LIB_PATH and not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(_lib_path)
         vp = C.c_void_p
         L.orc_create.restype = vp
         L.orc_create.argtypes = [C.POINTER(N.CModelDesc), C.c_int]
@@ -71,6 +97,10 @@ def lib():
         L.orc_singleton_tests.argtypes = [vp, N.p_i32, C.c_int64, N.p_u8, C.c_int, C.POINTER(OrcStats)]
         L.orc_mac.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(OrcStats), N.p_i32,
                               C.POINTER(N.CNodeRecord), C.c_int32, N.p_i32]
+        L.orc_mac_all.argtypes = [vp, C.c_int64, C.c_int64, C.c_int64, C.POINTER(OrcStats), N.p_i32,
+                                  C.c_int64, C.POINTER(N.CNodeRecord), C.c_int32, N.p_i32]
+        L.orc_bf_mac_all.argtypes = [C.POINTER(N.CModelDesc), C.c_int64, C.c_int64, C.POINTER(OrcStats),
+                                     N.p_i32, C.c_int64, C.POINTER(N.CNodeRecord), C.c_int32, N.p_i32]
         L.orc_dom_hash.restype = C.c_uint64
         L.orc_dom_hash.argtypes = [N.p_u64, C.c_int, C.c_int]
         L.orc_bf_gac.argtypes = [C.POINTER(N.CModelDesc), N.p_u64]
@@ -159,6 +189,51 @@ class Oracle:
                  for r in tr[:min(tl.value, trace_cap)]]
         return MacResult(int(st.num_sol), int(st.num_positive), int(st.num_negative),
                          bool(st.time_out), sol if st.num_sol else None, trace, st)
+
+
+@dataclass
+class MacAllResult:
+    num_sol: int
+    num_positive: int
+    num_negative: int
+    time_out: bool
+    solutions: np.ndarray  # [min(num_sol, sol_cap), n]
+    trace: List[Tuple[int, int, int, int, int]]
+
+
+def _mac_all_result(st, sols, n, cap, tr, tl, trace_cap):
+    k = min(int(st.num_sol), cap)
+    trace = [(r.positive, r.var, r.value, r.consistent, r.dom_hash)
+             for r in tr[:min(tl.value, trace_cap)]]
+    return MacAllResult(int(st.num_sol), int(st.num_positive), int(st.num_negative),
+                        bool(st.time_out), sols[:k * n].reshape(k, n), trace)
+
+
+def mac_all(o: "Oracle", max_solutions: int, node_limit: int = 0, sol_cap: int = 1024,
+            trace_cap: int = 0) -> MacAllResult:
+    """MAC that counts every solution up to max_solutions (orc_mac_all)."""
+    sols = np.full(max(1, sol_cap) * o.n, -1, np.int32)
+    tr = (N.CNodeRecord * max(1, trace_cap))()
+    tl = C.c_int32(0)
+    st = OrcStats()
+    lib().orc_mac_all(o.h, 0, node_limit, max_solutions, C.byref(st), _ptr(sols, C.c_int32), sol_cap,
+                      tr, trace_cap, C.byref(tl))
+    return _mac_all_result(st, sols, o.n, sol_cap, tr, tl, trace_cap)
+
+
+def bf_mac_all(desc, max_solutions: int, node_limit: int = 0, sol_cap: int = 1024,
+               trace_cap: int = 0) -> MacAllResult:
+    """The brute-force closure MAC with the same enumeration (orc_bf_mac_all)."""
+    c = desc.c_desc()
+    sols = np.full(max(1, sol_cap) * desc.num_vars, -1, np.int32)
+    tr = (N.CNodeRecord * max(1, trace_cap))()
+    tl = C.c_int32(0)
+    st = OrcStats()
+    rc = lib().orc_bf_mac_all(C.byref(c), node_limit, max_solutions, C.byref(st), _ptr(sols, C.c_int32),
+                              sol_cap, tr, trace_cap, C.byref(tl))
+    if rc < 0:
+        raise ValueError("the brute-force MAC has no dom/wdeg form")
+    return _mac_all_result(st, sols, desc.num_vars, sol_cap, tr, tl, trace_cap)
 
 
 def dom_hash(dom: np.ndarray) -> int:

@@ -702,8 +702,17 @@ int orc_singleton_tests(orc_net* net, const int32_t* tests, int64_t count, uint8
 // select_val VLH_MIN (:423-425); AssignedStack (Solver.cpp:36-116).  The
 // constraint weights live in the network and persist across calls, as
 // Tabular::weight does.
-int orc_mac(orc_net* net, int64_t time_limit_ms, int64_t node_limit, orc_stats* st,
-            int32_t* solution, cpim_node_record* trace, int32_t trace_cap, int32_t* trace_len) {
+//
+// Enumeration (max_solutions > 1) is not in the reference's MAC::enforce, which
+// stops at the first solution; its commented-out continuation (MAC.cpp:138-139,
+// "++sol_count_; consistent_ = false;") is what is done here: a solution is
+// counted and stored, then treated as a failed node, so the search backtracks
+// and goes on until the tree is exhausted or max_solutions are found.  This
+// is the search GModelSolver::SetFindAllSolutions / SetMaxSolutions ask for
+// (src/GModelSolver.cu:20-29, 125-128), in the CPU MAC's binary branching.
+int orc_mac_all(orc_net* net, int64_t time_limit_ms, int64_t node_limit, int64_t max_solutions,
+                orc_stats* st, int32_t* solutions, int64_t sol_cap, cpim_node_record* trace,
+                int32_t trace_cap, int32_t* trace_len) {
   Net& N = *net->N;
   AC3 ac(N);
   orc_stats s;
@@ -762,11 +771,14 @@ int orc_mac(orc_net* net, int64_t time_limit_ms, int64_t node_limit, orc_stats* 
       ++s.fixpoints;
       record(1, var, val, consistent, (int)I.size());
       if (consistent && (int)I.size() == N.n) {
-        finished = true;
-        s.num_sol = 1;
-        if (solution)
-          for (int v = 0; v < N.n; ++v) solution[v] = N.head(v, (int)I.size());
-        break;
+        if (solutions && s.num_sol < sol_cap)
+          for (int v = 0; v < N.n; ++v) solutions[s.num_sol * N.n + v] = N.head(v, (int)I.size());
+        ++s.num_sol;
+        if (max_solutions <= 1 || s.num_sol >= max_solutions) {
+          finished = true;
+          break;
+        }
+        consistent = false;  // go on: backtrack as after a failed node
       }
       while (!consistent && !I.empty()) {
         const auto va = I.back();
@@ -790,6 +802,12 @@ int orc_mac(orc_net* net, int64_t time_limit_ms, int64_t node_limit, orc_stats* 
   if (st) *st = s;
   if (trace_len) *trace_len = tl;
   return (int)s.num_sol;
+}
+
+int orc_mac(orc_net* net, int64_t time_limit_ms, int64_t node_limit, orc_stats* st,
+            int32_t* solution, cpim_node_record* trace, int32_t trace_cap, int32_t* trace_len) {
+  return orc_mac_all(net, time_limit_ms, node_limit, 1, st, solution, solution ? 1 : 0, trace, trace_cap,
+                     trace_len);
 }
 
 uint64_t orc_dom_hash(const uint64_t* dom, int n, int words) {
@@ -885,9 +903,9 @@ int64_t orc_bf_count_solutions(const cpim_model_desc* desc, int64_t limit, int32
   return count;
 }
 
-int orc_bf_mac(const cpim_model_desc* desc, int64_t node_limit, orc_stats* st,
-               int32_t* solution, cpim_node_record* trace, int32_t trace_cap,
-               int32_t* trace_len) {
+int orc_bf_mac_all(const cpim_model_desc* desc, int64_t node_limit, int64_t max_solutions, orc_stats* st,
+                   int32_t* solutions, int64_t sol_cap, cpim_node_record* trace, int32_t trace_cap,
+                   int32_t* trace_len) {
   // dom/wdeg has no brute-force counterpart: its weights follow AC3's order
   if (g_varh == 2) return -1;
   BfModel M = bf_model(desc);
@@ -932,14 +950,15 @@ int orc_bf_mac(const cpim_model_desc* desc, int64_t node_limit, orc_stats* st,
     ok = bf_close(M, D);
     record(1, x, a, ok, D);
     if (ok && (int)stack.size() == M.n) {
-      s.num_sol = 1;
-      if (solution)
+      if (solutions && s.num_sol < sol_cap)
         for (int v = 0; v < M.n; ++v) {
           int b = 0;
           while (!D[v][b]) ++b;
-          solution[v] = b;
+          solutions[s.num_sol * M.n + v] = b;
         }
-      break;
+      ++s.num_sol;
+      if (max_solutions <= 1 || s.num_sol >= max_solutions) break;
+      ok = false;  // go on: backtrack as after a failed node
     }
     while (!ok && !stack.empty()) {
       Frame f = std::move(stack.back());
@@ -956,6 +975,12 @@ int orc_bf_mac(const cpim_model_desc* desc, int64_t node_limit, orc_stats* st,
   if (st) *st = s;
   if (trace_len) *trace_len = tl;
   return (int)s.num_sol;
+}
+
+int orc_bf_mac(const cpim_model_desc* desc, int64_t node_limit, orc_stats* st,
+               int32_t* solution, cpim_node_record* trace, int32_t trace_cap,
+               int32_t* trace_len) {
+  return orc_bf_mac_all(desc, node_limit, 1, st, solution, solution ? 1 : 0, trace, trace_cap, trace_len);
 }
 
 }  // extern "C"

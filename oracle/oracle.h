@@ -80,6 +80,14 @@ int orc_singleton_tests(orc_net* net, const int32_t* tests, int64_t count, uint8
 int orc_mac(orc_net* net, int64_t time_limit_ms, int64_t node_limit, orc_stats* st,
             int32_t* solution, cpim_node_record* trace, int32_t trace_cap, int32_t* trace_len);
 
+/* orc_mac with enumeration: every solution is counted (and stored in
+ * solutions[k * n .. k * n + n) while k < sol_cap), then treated as a failed
+ * node so the search goes on, until the tree is exhausted or max_solutions
+ * are found (max_solutions <= 1: stop at the first, as orc_mac). */
+int orc_mac_all(orc_net* net, int64_t time_limit_ms, int64_t node_limit, int64_t max_solutions,
+                orc_stats* st, int32_t* solutions, int64_t sol_cap, cpim_node_record* trace,
+                int32_t trace_cap, int32_t* trace_len);
+
 /* Order-free domain hash of one level (same definition as cpim_node_record). */
 uint64_t orc_dom_hash(const uint64_t* dom, int n, int words);
 
@@ -98,6 +106,10 @@ int64_t orc_bf_count_solutions(const cpim_model_desc* desc, int64_t limit, int32
 int orc_bf_mac(const cpim_model_desc* desc, int64_t node_limit, orc_stats* st,
                int32_t* solution, cpim_node_record* trace, int32_t trace_cap,
                int32_t* trace_len);
+/* orc_bf_mac with the enumeration of orc_mac_all. */
+int orc_bf_mac_all(const cpim_model_desc* desc, int64_t node_limit, int64_t max_solutions, orc_stats* st,
+                   int32_t* solutions, int64_t sol_cap, cpim_node_record* trace, int32_t trace_cap,
+                   int32_t* trace_len);
 
 /* Planted faults for pin tests (0 = none):
  *  1: AC3bit seek_support ignores the last word of the other domain

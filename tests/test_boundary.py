@@ -39,7 +39,7 @@ def test_struct_layouts_match_header_sizes():
     assert C.sizeof(N.CNodeRecord) == 24
     assert C.sizeof(N.CSearchStats) == 64
     assert C.sizeof(N.CBatchResult) == 32
-    assert C.sizeof(N.CSolveOpts) == 32
+    assert C.sizeof(N.CSolveOpts) == 48
 
 
 def test_version_and_default_options():

@@ -622,3 +622,65 @@ def test_mac_stops_at_an_empty_unconstrained_domain():
         r = oracle.Oracle(desc).mac(trace_cap=64)
     # dom-min takes the empty domain first: no node at all
     assert (r.num_positive, r.num_negative, r.trace) == (0, 0, [])
+
+
+# ---------------------------------------------------------------------------
+# Enumeration (find-all / max-solutions): a found solution is counted, then
+# treated as a failed node.  Pinned by exhaustive enumeration written here
+# (every assignment checked against the tuple lists), by the solution count
+# of the oracle's brute-force enumerator, and by the closure-per-node MAC
+# driver with the same enumeration (identical node counts, traces and
+# solution order).
+# ---------------------------------------------------------------------------
+def all_solutions_python(desc):
+    doms = domains_to_sets(desc.initial_domains(), desc.dom_size)
+    return sorted(tuple(a) for a in itertools.product(*doms) if check_solution(desc, a))
+
+
+def test_find_all_queens4():
+    desc, _ = load_golden("queens-4_ext")
+    r = oracle.mac_all(oracle.Oracle(desc), 10**6, trace_cap=256)
+    assert r.num_sol == 2
+    assert sorted(map(tuple, r.solutions.tolist())) == [(1, 3, 0, 2), (2, 0, 3, 1)]
+    # the first solution is the one MAC stops at
+    assert list(r.solutions[0]) == [1, 3, 0, 2]
+    b = oracle.bf_mac_all(desc, 10**6, trace_cap=256)
+    assert (r.num_positive, r.num_negative, r.trace) == (b.num_positive, b.num_negative, b.trace)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_find_all_vs_exhaustive_enumeration(seed):
+    n, d, e, p = [(6, 3, 8, 0.3), (7, 3, 10, 0.35), (5, 4, 7, 0.4), (8, 2, 6, 0.2)][seed % 4]
+    desc = workload.random_binary(n, d, e, p, 900 + seed)
+    want = all_solutions_python(desc)
+    r = oracle.mac_all(oracle.Oracle(desc), 10**6, sol_cap=4096, trace_cap=4096)
+    assert r.num_sol == len(want) == oracle.bf_count_solutions(desc)[0]
+    got = list(map(tuple, r.solutions.tolist()))
+    assert len(set(got)) == len(got) and sorted(got) == want
+    b = oracle.bf_mac_all(desc, 10**6, sol_cap=4096, trace_cap=4096)
+    assert (r.num_positive, r.num_negative) == (b.num_positive, b.num_negative)
+    assert r.trace == b.trace and (r.solutions == b.solutions).all()
+
+
+@pytest.mark.parametrize("k", [1, 2, 5])
+def test_max_solutions_stops_at_k(k):
+    desc = workload.random_binary(7, 3, 8, 0.3, 77)
+    full = oracle.mac_all(oracle.Oracle(desc), 10**6, sol_cap=4096)
+    assert full.num_sol > 5
+    r = oracle.mac_all(oracle.Oracle(desc), k, sol_cap=4096, trace_cap=4096)
+    assert r.num_sol == k and (r.solutions == full.solutions[:k]).all()
+    b = oracle.bf_mac_all(desc, k, sol_cap=4096, trace_cap=4096)
+    assert (r.num_positive, r.num_negative, r.trace) == (b.num_positive, b.num_negative, b.trace)
+    if k == 1:
+        one = oracle.Oracle(desc).mac(trace_cap=4096)
+        assert (one.num_positive, one.num_negative, one.trace) == (r.num_positive, r.num_negative, r.trace)
+
+
+def test_find_all_unsat_and_capacity():
+    desc, _ = load_golden("composed-25-1-2-2_ext")
+    r = oracle.mac_all(oracle.Oracle(desc), 10**6)
+    assert r.num_sol == 0
+    # more solutions than the buffer holds: all counted, the first sol_cap kept
+    desc = workload.random_binary(8, 2, 4, 0.0, 3)
+    r = oracle.mac_all(oracle.Oracle(desc), 10**6, sol_cap=10)
+    assert r.num_sol == 2 ** 8 and r.solutions.shape == (10, 8)

@@ -2,12 +2,13 @@
 reader equal the golden models, and the normalization rules of
 src/model/model_normalizer.cpp hold on hand-made instances."""
 import os
+import shutil
 
 import numpy as np
 import pytest
 
 from conftest import load_golden
-from cpim_amd import xcsp
+from cpim_amd import tier, xcsp
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 XML = os.path.join(HERE, "golden", "xcsp")
@@ -138,3 +139,70 @@ def test_tier_reports_unreadable_files(tmp_path):
     r = tier.run_one(str(tmp_path / "missing.xml"), timeout_s=20)
     assert r.status.startswith("ERROR"), r
     assert tier.main([str(bad), "--quiet"]) == 1
+
+
+# ---------------------------------------------------------------------------
+# Bench manifests (BMPath.xml, src/model/libxml2_parser.cpp:88-116, 165-383)
+# ---------------------------------------------------------------------------
+GOLDEN_XCSP = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "xcsp")
+
+
+def manifest(entries):
+    body = "".join(f'<BMFile id="{i}"{(" type=%s" % repr(t)) if t else ""}>{p}</BMFile>'
+                   for i, (p, t) in enumerate(entries))
+    return f'<?xml version="1.0"?><BenchMark><BMFiles nbBMFiles="{len(entries)}">{body}</BMFiles></BenchMark>'
+
+
+def test_manifest_in_the_reference_layout(tmp_path):
+    # samples/bench/BMPath.xml names ../samples/bench/<file>: relative to the
+    # manifest's grandparent
+    bench = tmp_path / "samples" / "bench"
+    bench.mkdir(parents=True)
+    shutil.copy(os.path.join(GOLDEN_XCSP, "queens-12_ext.xml"), bench / "queens-12_ext.xml")
+    m = bench / "BMPath.xml"
+    m.write_text(manifest([("../samples/bench/queens-12_ext.xml", None)]))
+    entries = xcsp.load_bench_manifest(str(m))
+    assert len(entries) == 1 and entries[0].kind == "file"
+    assert entries[0].files[0].path == str((bench / "queens-12_ext.xml").resolve())
+    inst = xcsp.read(str(m))
+    direct = xcsp.read(str(bench / "queens-12_ext.xml"))
+    assert inst.num_vars == direct.num_vars == 12 and inst.num_cons == direct.num_cons
+
+
+def test_manifest_directory_entries_and_tier_expansion(tmp_path):
+    d = tmp_path / "suite"
+    (d / "sub").mkdir(parents=True)
+    shutil.copy(os.path.join(GOLDEN_XCSP, "queens-4_ext.xml"), d / "b.xml")
+    shutil.copy(os.path.join(GOLDEN_XCSP, "test.xml"), d / "sub" / "a.xml")
+    shutil.copy(os.path.join(GOLDEN_XCSP, "queens-12_ext.xml"), tmp_path / "q12.xml")
+    m = tmp_path / "BMPath.xml"
+    m.write_text(manifest([("suite", "dir"), ("q12.xml", None), ("  ", None)]))
+    entries = xcsp.load_bench_manifest(str(m))
+    assert [e.kind for e in entries] == ["dir", "file"]
+    assert [os.path.basename(f.path) for f in entries[0].files] == ["b.xml", "a.xml"]
+    assert all(f.format for e in entries for f in e.files)
+    # the parser reads the first file of the first entry
+    assert xcsp.read(str(m)).num_vars == 4
+    assert [os.path.basename(p) for p in tier.expand_paths([str(m)])] == ["b.xml", "a.xml", "q12.xml"]
+
+
+@pytest.mark.parametrize("case", ["no_section", "empty", "missing", "type_mismatch"])
+def test_manifest_errors(tmp_path, case):
+    shutil.copy(os.path.join(GOLDEN_XCSP, "test.xml"), tmp_path / "t.xml")
+    m = tmp_path / "BMPath.xml"
+    if case == "no_section":
+        m.write_text('<BenchMark><BMFile id="0">t.xml</BMFile></BenchMark>')
+        with pytest.raises(xcsp.XcspError, match="missing <BMFiles>"):
+            xcsp.load_bench_manifest(str(m))
+    elif case == "empty":
+        m.write_text(manifest([("", None)]))
+        with pytest.raises(xcsp.XcspError, match="did not contain usable"):
+            xcsp.load_bench_manifest(str(m))
+    elif case == "missing":
+        m.write_text(manifest([("nowhere.xml", None)]))
+        with pytest.raises(FileNotFoundError, match="Bench path not found"):
+            xcsp.read(str(m))
+    else:
+        m.write_text(manifest([("t.xml", "dir")]))
+        with pytest.raises(xcsp.XcspError, match="expected directory but resolved to file"):
+            xcsp.load_bench_manifest(str(m))
