@@ -196,6 +196,7 @@ GPU_SIGNATURES = {
     "cpim_gmodel_destroy": (C.c_int, [_VP]),
     "cpim_gmodel_get_info": (C.c_int, [_VP, C.POINTER(CGModelInfo)]),
     "cpim_gmodel_enforce_gac": (C.c_int, [_VP, C.c_int32, C.c_int32, C.POINTER(CGacStats)]),
+    "cpim_gmodel_enforce_gac_noalloc": (C.c_int, [_VP, C.c_int32, C.POINTER(CGacStats)]),
     "cpim_gmodel_create_new_level": (C.c_int, [_VP, p_i32]),
     "cpim_gmodel_back_to_level": (C.c_int, [_VP, C.c_int32]),
     "cpim_gmodel_current_level": (C.c_int, [_VP, p_i32]),

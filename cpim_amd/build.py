@@ -20,7 +20,7 @@ WORKLOAD_LIB = os.path.join(HERE, "libcpim_workload.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 GPU_SOURCES = ["engine.hip"]
-GPU_DEPS = ["engine.hip", "batch.inc", "kernels.h", "device.h"]
+GPU_DEPS = ["engine.hip", "batch.inc", "noalloc.inc", "kernels.h", "device.h"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

@@ -1825,3 +1825,5 @@ int cpim_gmodel_solve(cpim_gmodel* m, const cpim_solve_opts* opts, cpim_search_s
 }  // extern "C"
 
 #include "batch.inc"
+
+#include "noalloc.inc"

@@ -167,6 +167,16 @@ typedef struct cpim_gac_stats {
 int cpim_gmodel_enforce_gac(cpim_gmodel* model, int32_t assigned_var, int32_t mode,
                             cpim_gac_stats* stats);
 
+/* No counterpart in the reference.  The persistent mode of
+ * cpim_gmodel_enforce_gac for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): it
+ * allocates nothing.  Its scratch (control block and frontier bitmaps) is a
+ * file-scope device array of 769 KiB per GPU, held until the process ends, so
+ * it takes networks of at most 2,097,152 propagated constraints
+ * (CPIM_ERR_INVALID_ARGUMENT above that).  Calls that overlap on one GPU race
+ * on that array.  stats->kernel_ms is 0 (no events are created). */
+int cpim_gmodel_enforce_gac_noalloc(cpim_gmodel* model, int32_t assigned_var, cpim_gac_stats* stats);
+
 /* Level / trail API (include/GModel.cuh:219-253, src/GModel.cu:1397-1701),
  * same return conventions: results are written to *out. */
 int cpim_gmodel_create_new_level(cpim_gmodel* model, int32_t* new_level);
