@@ -49,11 +49,15 @@ def parse():
                     help="pass kernel: 64 bit-sliced tests per workgroup, or one test per workgroup")
     ap.add_argument("--skip-extras", action="store_true",
                     help="skip the single-fixpoint, uneven-batch and serving measurements")
-    ap.add_argument("--batch-instances", type=int, default=512,
+    ap.add_argument("--batch-instances", type=int, default=4096,
                     help="instances of the 4096-instance uneven batch timed in the extras")
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--serve-rate", type=float, default=2000.0,
                     help="Poisson arrival rate of serving requests (requests/s)")
+    ap.add_argument("--serve-c3-requests", type=int, default=48,
+                    help="requests of the benchmark SAC network's shape (with tables) served in the extras")
+    ap.add_argument("--serve-c3-rate", type=float, default=40.0,
+                    help="Poisson arrival rate of those requests (requests/s)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (read MIB + write MIB) and exit: the "
                          "reference kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE counters")
@@ -179,45 +183,70 @@ def single_fixpoint(log) -> dict:
     }
 
 
-def uneven_batch(count: int, log) -> dict:
-    """Root fixpoint + first 64 MAC nodes of `count` instances of the uneven batch."""
-    from cpim_amd import Batch, workload
+def uneven_batch(count: int, log, per_instance_sample: int = 256) -> dict:
+    """Root fixpoint + first 64 MAC nodes of `count` instances of the uneven
+    batch in one launch (units heaviest first, the unit shape chosen by the
+    autotuner on this batch before the timed run), against one launch per
+    instance timed on a spread sample of the same instances."""
+    from concurrent.futures import ThreadPoolExecutor
+    from cpim_amd import Batch, autotune, workload
     t = time.perf_counter()
-    models = [workload.uneven_instance(i) for i in range(count)]
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        models = list(ex.map(workload.uneven_instance, range(count)))
     log(f"uneven batch: {count} instances generated in {time.perf_counter() - t:.1f} s")
+    t = time.perf_counter()
     b = Batch(models)
+    log(f"uneven batch: uploaded in {time.perf_counter() - t:.1f} s")
     try:
+        tt = time.perf_counter()
+        choice = autotune.tuned_batch(b, 64)
+        tune_s = time.perf_counter() - tt
         b.run(64)  # warm-up
         t = time.perf_counter()
         res, kms = b.run(64, persistent=True)
         one = time.perf_counter() - t
-        t = time.perf_counter()
-        b.run(64, persistent=False)
-        per = time.perf_counter() - t
     finally:
         b.close()
     nodes = sum(r.num_positive + r.num_negative for r in res)
-    out = {"workload": f"{count} of the 4096 uneven instances (n~U[20,200], d in {{8..128}}, "
-                       "density~U[0.1,0.5], phase-transition tightness): root fixpoint + first 64 "
-                       "MAC nodes each",
+    # one launch per instance, on a spread sample
+    idx = np.linspace(0, count - 1, min(per_instance_sample, count)).astype(np.int64)
+    sub = [models[i] for i in idx]
+    bs = Batch(sub)
+    try:
+        bs.run(64, persistent=False)
+        t = time.perf_counter()
+        rs, _ = bs.run(64, persistent=False)
+        per = time.perf_counter() - t
+    finally:
+        bs.close()
+    for i, r in zip(idx, rs):
+        a = res[i]
+        assert (a.status, a.num_positive, a.num_negative) == (r.status, r.num_positive, r.num_negative), \
+            "one launch and one launch per instance disagree"
+    sub_fix = len(sub) + sum(r.num_positive + r.num_negative for r in rs)
+    out = {"workload": f"{count} uneven instances (n~U[20,200], d in {{8..128}}, density~U[0.1,0.5], "
+                       "phase-transition tightness): root fixpoint + first 64 MAC nodes each",
            "instances": count, "nodes": int(nodes),
-           "one_launch_ms": 1000.0 * one, "per_instance_ms": 1000.0 * per,
+           "one_launch_ms": 1000.0 * one, "kernel_ms": kms,
            "fixpoints_per_s": (count + nodes) / one,
-           "speedup_vs_per_instance": per / one}
-    log(f"uneven batch: one launch {out['one_launch_ms']:.1f} ms, per instance "
-        f"{out['per_instance_ms']:.1f} ms, {out['fixpoints_per_s']:,.0f} fixpoints/s")
+           "tuned_choice": choice, "tuning_s": tune_s,
+           "per_instance_sample": len(sub),
+           "per_instance_fixpoints_per_s": sub_fix / per,
+           "speedup_vs_per_instance": ((count + nodes) / one) / (sub_fix / per)}
+    log(f"uneven batch: one launch {out['one_launch_ms']:.1f} ms ({out['fixpoints_per_s']:,.0f} fixpoints/s), "
+        f"per instance {out['per_instance_fixpoints_per_s']:,.0f} fixpoints/s, tuned {choice}")
     return out
 
 
-def serving(count: int, rate: float, log) -> dict:
-    """Poisson arrivals of MAC solve requests (binary part + arity-3 tables)."""
-    from cpim_amd import Server, workload
-    node_limit = 5000
-    reqs = [workload.serving_request(i) for i in range(count)]
+def serve_poisson(reqs, rate: float, node_limit: int, nodes_per_step: int, log, label: str) -> dict:
+    """Poisson arrivals of the prepared requests at `rate`/s into one server:
+    solves/s over the whole run, latency p50 / p99 from arrival to result."""
+    from cpim_amd import Server
+    count = len(reqs)
     rng = np.random.default_rng(0x5EED2026)
     arrive = np.cumsum(rng.exponential(1.0 / rate, count))
-    srv = Server(max_slots=256, nodes_per_step=64)
-    lat, limited, solved = [], 0, 0
+    srv = Server(max_slots=256, nodes_per_step=nodes_per_step)
+    lat, limited, solved, unsat = [], 0, 0, 0
     try:
         pending = {}
         nxt = 0
@@ -234,21 +263,46 @@ def serving(count: int, rate: float, log) -> dict:
             done_at = time.perf_counter() - t0
             for rid in srv.poll():
                 r = srv.result(rid)
+                srv.release(rid)
                 lat.append(done_at - pending.pop(rid))
                 limited += r["status"] == 0
                 solved += r["status"] == 1
+                unsat += r["status"] == 2
         wall = time.perf_counter() - t0
+        peak = srv.memory()["peak_bytes"]
     finally:
         srv.close()
     lat_ms = 1000.0 * np.array(lat)
-    out = {"workload": f"{count} requests, Poisson arrivals at {rate:g}/s; each a MAC solve "
-                       f"(node limit {node_limit}) of n~U[20,40] d=16 binary CSP + 4 arity-3 "
-                       "tables of 500 tuples",
-           "solves_per_s": count / wall, "p50_ms": float(np.percentile(lat_ms, 50)),
-           "p99_ms": float(np.percentile(lat_ms, 99)), "solved": int(solved),
-           "node_limited": int(limited)}
-    log(f"serving: {out['solves_per_s']:.0f} solves/s, p50 {out['p50_ms']:.1f} ms, "
+    out = {"solves_per_s": count / wall, "offered_per_s": rate, "p50_ms": float(np.percentile(lat_ms, 50)),
+           "p99_ms": float(np.percentile(lat_ms, 99)), "solved": int(solved), "unsat": int(unsat),
+           "node_limited": int(limited), "peak_device_bytes": int(peak)}
+    log(f"serving [{label}]: {out['solves_per_s']:.1f} solves/s offered {rate:g}/s, p50 {out['p50_ms']:.1f} ms, "
         f"p99 {out['p99_ms']:.1f} ms")
+    return out
+
+
+def serving(count: int, rate: float, c3_count: int, c3_rate: float, log) -> dict:
+    """Poisson arrivals of MAC solve requests: small binary CSPs with arity-3
+    tables, and requests of the benchmark SAC network's shape with tables
+    under a node budget."""
+    from concurrent.futures import ThreadPoolExecutor
+    from cpim_amd import workload
+    out = {}
+    reqs = [workload.serving_request(i) for i in range(count)]
+    small = serve_poisson(reqs, rate, 5000, 64, log, "small")
+    small["workload"] = (f"{count} requests, Poisson arrivals at {rate:g}/s; each a MAC solve (node limit "
+                         "5000) of n~U[20,40] d=16 binary CSP + 4 arity-3 tables of 500 tuples")
+    out["small"] = small
+    if c3_count > 0:
+        t = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+            reqs = list(ex.map(workload.serving_request_c3, range(c3_count)))
+        log(f"serving [c3]: {c3_count} requests generated in {time.perf_counter() - t:.1f} s")
+        c3 = serve_poisson(reqs, c3_rate, 64, 16, log, "c3")
+        c3["workload"] = (f"{c3_count} requests, Poisson arrivals at {c3_rate:g}/s; each a MAC search with a "
+                          "budget of 64 nodes on a random binary CSP n=500 d=256 e=20000 tightness 0.5 + 4 "
+                          "arity-3 tables of 500 tuples over 16 values")
+        out["c3_shaped"] = c3
     return out
 
 
@@ -380,7 +434,8 @@ def main():
     if not args.skip_extras:
         for key, fn in (("single_fixpoint", lambda: single_fixpoint(log)),
                         ("uneven_batch", lambda: uneven_batch(args.batch_instances, log)),
-                        ("serving", lambda: serving(args.serve_requests, args.serve_rate, log))):
+                        ("serving", lambda: serving(args.serve_requests, args.serve_rate,
+                                                    args.serve_c3_requests, args.serve_c3_rate, log))):
             try:
                 out[key] = fn()
             except Exception as exc:  # reported in the line, never hidden

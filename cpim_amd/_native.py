@@ -228,6 +228,9 @@ GPU_SIGNATURES = {
     "cpim_tune_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64)]),
     "cpim_server_poll": (C.c_int, [_VP, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32)]),
     "cpim_server_result": (C.c_int, [_VP, C.c_int64, C.POINTER(CRequestResult), p_i32]),
+    "cpim_server_release": (C.c_int, [_VP, C.c_int64]),
+    "cpim_server_memory": (C.c_int, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32)]),
 }
 
 _gpu = None

@@ -217,6 +217,24 @@ __device__ __forceinline__ int wave_scan_dpp(int v, int* total) {
 // update other workgroups (on any XCD) have completed; used by the
 // asynchronous fixpoint, which has no barrier between rounds.
 template <bool kCoherent>
+__device__ __forceinline__ u64 ld_dom(const u64* p);
+
+// Loads and read-modify-writes of a global buffer that only one workgroup
+// touches (the bit-sliced test state): workgroup scope is enough, and the
+// __syncthreads() barriers between phases order these accesses.  An
+// agent-scope fence (__threadfence) would write back and invalidate the whole
+// L2 of the XCD instead; an agent-scope load bypasses it.
+__device__ __forceinline__ u64 ld_wg(const u64* p) {
+  return __hip_atomic_load(const_cast<u64*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ u64 or_wg(u64* p, u64 v) {
+  return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ u64 and_wg(u64* p, u64 v) {
+  return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <bool kCoherent>
 __device__ __forceinline__ u64 ld_dom(const u64* p) {
   if constexpr (kCoherent) {
     return __hip_atomic_load(const_cast<u64*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

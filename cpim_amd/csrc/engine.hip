@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -294,29 +295,63 @@ ModelImage make_image(const cpim_model_desc* d, bool skip_non_binary, bool own_r
 // ---------------------------------------------------------------------------
 // Device copy of a network
 // ---------------------------------------------------------------------------
+// Device memory: owned (freed on destruction) or a view into an Arena.
 struct DevAlloc {
   void* p = nullptr;
+  bool owned = true;
   DevAlloc() = default;
   explicit DevAlloc(size_t bytes) {
     if (bytes == 0) bytes = 16;
     HIPC(hipMalloc(&p, bytes));
   }
+  static DevAlloc view(void* q) {
+    DevAlloc d;
+    d.p = q;
+    d.owned = false;
+    return d;
+  }
   ~DevAlloc() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
   }
   DevAlloc(const DevAlloc&) = delete;
   DevAlloc& operator=(const DevAlloc&) = delete;
-  DevAlloc(DevAlloc&& o) noexcept : p(o.p) { o.p = nullptr; }
+  DevAlloc(DevAlloc&& o) noexcept : p(o.p), owned(o.owned) { o.p = nullptr; }
   DevAlloc& operator=(DevAlloc&& o) noexcept {
     if (this != &o) {
-      if (p) (void)hipFree(p);
+      if (p && owned) (void)hipFree(p);
       p = o.p;
+      owned = o.owned;
       o.p = nullptr;
     }
     return *this;
   }
   template <typename T>
   T* as() const { return static_cast<T*>(p); }
+};
+
+// One device allocation carved into the arrays of one network (and its
+// search state): a served request costs one hipMalloc instead of one per
+// array.  plan mode (block == nullptr) only adds up the sizes.
+struct Arena {
+  DevAlloc block;
+  long long cap = 0, used = 0;
+  static long long round(long long b) { return (std::max<long long>(b, 16) + 255) & ~255ll; }
+  void reserve(long long bytes) {
+    block = DevAlloc((size_t)std::max<long long>(bytes, 256));
+    cap = bytes;
+    used = 0;
+  }
+  DevAlloc take(long long bytes) {
+    const long long b = round(bytes);
+    if (!block.p) {  // plan mode
+      used += b;
+      return DevAlloc();
+    }
+    if (used + b > cap) fail(CPIM_ERR_RUNTIME, "arena overflow (size plan and build disagree)");
+    DevAlloc d = DevAlloc::view(static_cast<char*>(block.p) + used);
+    used += b;
+    return d;
+  }
 };
 
 struct DevModel {
@@ -340,10 +375,15 @@ struct DevModel {
   }
 };
 
+// One device array: a view of `arena` when given, else its own allocation.
+DevAlloc dev_array(Arena* arena, long long bytes) {
+  return arena ? arena->take(bytes) : DevAlloc((size_t)std::max<long long>(bytes, 16));
+}
+
 template <typename T>
-void upload(DevAlloc& dst, const std::vector<T>& src, hipStream_t s, long long& bytes) {
+void upload(DevAlloc& dst, const std::vector<T>& src, hipStream_t s, long long& bytes, Arena* arena = nullptr) {
   const size_t b = std::max<size_t>(sizeof(T) * src.size(), 16);
-  dst = DevAlloc(b);
+  dst = dev_array(arena, (long long)b);
   if (!src.empty()) HIPC(hipMemcpyAsync(dst.p, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, s));
   bytes += (long long)b;
 }
@@ -358,18 +398,32 @@ long long align16(long long b) { return (b + 15) & ~15ll; }
 // planes (the plane rows are copied from the device planes); skipped when the
 // image would exceed kLdsNetMax.  The copies are queued on s from `h`,
 // which the caller keeps until s is synchronized.
-void build_lds_image(const ModelImage& im, DevModel& dm, hipStream_t s, std::vector<char>& h) {
-  const int n = im.n, W = im.W, dpad = im.dpad, dc = std::max(1, im.max_dom);
+struct LdsLayout {
+  long long o_adj, o_con, o_sup, o_tsup, bytes;
+};
+
+LdsLayout lds_layout(const ModelImage& im) {
+  const int n = im.n, W = im.W, dc = std::max(1, im.max_dom);
   const long long npairs = (long long)im.bin_cons.size();
   long long tsup_c = 0;
   for (const ConMeta& m : im.con)
     if (m.kind == 1) tsup_c += (long long)m.arity * dc * m.tw;
-  const long long o_adj = align16(4ll * (n + 1));
-  const long long o_con = o_adj + align16(16ll * (long long)im.adj.size());
-  const long long o_sup = o_con + align16((long long)sizeof(ConMeta) * (long long)im.con.size());
+  LdsLayout L;
+  L.o_adj = align16(4ll * (n + 1));
+  L.o_con = L.o_adj + align16(16ll * (long long)im.adj.size());
+  L.o_sup = L.o_con + align16((long long)sizeof(ConMeta) * (long long)im.con.size());
   const long long sup_words = npairs * 2 * W * dc + 64ll * W;
-  const long long o_tsup = o_sup + align16(8 * sup_words);
-  const long long bytes = o_tsup + align16(8 * std::max<long long>(1, tsup_c));
+  L.o_tsup = L.o_sup + align16(8 * sup_words);
+  L.bytes = L.o_tsup + align16(8 * std::max<long long>(1, tsup_c));
+  return L;
+}
+
+void build_lds_image(const ModelImage& im, DevModel& dm, hipStream_t s, std::vector<char>& h,
+                     Arena* arena = nullptr) {
+  const int n = im.n, W = im.W, dpad = im.dpad, dc = std::max(1, im.max_dom);
+  const long long npairs = (long long)im.bin_cons.size();
+  const LdsLayout L = lds_layout(im);
+  const long long o_adj = L.o_adj, o_con = L.o_con, o_sup = L.o_sup, o_tsup = L.o_tsup, bytes = L.bytes;
   if (bytes > kLdsNetMax) return;
   h.assign((size_t)bytes, 0);
   std::memcpy(h.data(), im.adj_off.data(), 4 * (size_t)(n + 1));
@@ -391,7 +445,7 @@ void build_lds_image(const ModelImage& im, DevModel& dm, hipStream_t s, std::vec
     }
     con[c] = m;
   }
-  dm.lnet_img = DevAlloc((size_t)bytes);
+  dm.lnet_img = dev_array(arena, bytes);
   HIPC(hipMemcpyAsync(dm.lnet_img.p, h.data(), (size_t)bytes, hipMemcpyHostToDevice, s));
   if (npairs > 0)
     HIPC(hipMemcpy2DAsync((char*)dm.lnet_img.p + o_sup, 8 * (size_t)dc, dm.sup.p, 8 * (size_t)dpad,
@@ -406,45 +460,79 @@ void build_lds_image(const ModelImage& im, DevModel& dm, hipStream_t s, std::vec
   dm.lnet.o_tsup = (int)o_tsup;
 }
 
-void build_device_model(const ModelImage& im, DevModel& dm, hipStream_t s) {
+// Device bytes build_device_model takes from an arena for im (the same
+// arrays, each rounded as the arena rounds it).
+long long model_arena_bytes(const ModelImage& im) {
+  auto r = [](long long b) { return Arena::round(std::max<long long>(b, 16)); };
+  long long b = r(4ll * (long long)im.adj_off.size()) + r(16ll * (long long)im.adj.size()) +
+                r((long long)sizeof(ConMeta) * (long long)im.con.size()) + r(8ll * (long long)im.tsup.size()) +
+                r(4ll * (long long)im.cap.size()) + r(4ll * (long long)im.nbr.size()) + r(8ll * im.sup_words);
+  const long long img = lds_layout(im).bytes;
+  if (img <= kLdsNetMax) b += r(img);
+  return b;
+}
+
+// Staging of the dense rows on their way to the planes: bins (the binary
+// constraints' indices) and slices of rows, reused across builds when the
+// caller keeps it.
+struct PlaneStaging {
+  DevAlloc buf;
+  long long bytes = 0;
+  char* get(long long b) {
+    if (b > bytes) {
+      buf = DevAlloc((size_t)b);
+      bytes = b;
+    }
+    return buf.as<char>();
+  }
+};
+
+// The device arrays of a network; with an arena they are views into it (one
+// allocation in all), with staging the row slices reuse its buffer.  Returns
+// after the planes and the image are complete on stream s.
+void build_device_model(const ModelImage& im, DevModel& dm, hipStream_t s, Arena* arena = nullptr,
+                        PlaneStaging* staging = nullptr) {
   dm.n = im.n;
   dm.W = im.W;
   dm.dpad = im.dpad;
   dm.max_dom = im.max_dom;
   dm.num_active = im.num_active;
-  upload(dm.adj_off, im.adj_off, s, dm.bytes);
-  upload(dm.adj, im.adj, s, dm.bytes);
-  upload(dm.con, im.con, s, dm.bytes);
-  upload(dm.tsup, im.tsup, s, dm.bytes);
-  upload(dm.cap, im.cap, s, dm.bytes);
-  upload(dm.nbr, im.nbr, s, dm.bytes);
-  dm.sup = DevAlloc(sizeof(u64) * im.sup_words);
+  upload(dm.adj_off, im.adj_off, s, dm.bytes, arena);
+  upload(dm.adj, im.adj, s, dm.bytes, arena);
+  upload(dm.con, im.con, s, dm.bytes, arena);
+  upload(dm.tsup, im.tsup, s, dm.bytes, arena);
+  upload(dm.cap, im.cap, s, dm.bytes, arena);
+  upload(dm.nbr, im.nbr, s, dm.bytes, arena);
+  dm.sup = dev_array(arena, (long long)sizeof(u64) * im.sup_words);
   dm.bytes += sizeof(u64) * im.sup_words;
   const int nb = (int)im.bin_cons.size();
   if (nb > 0) {
     // Upload the dense rows in slices and build the planes on the device.
-    DevAlloc bins;
-    long long dummy = 0;
-    upload(bins, im.bin_cons, s, dummy);
     const long long row_words = (long long)im.max_dom * im.rel_words;
     const int slice = (int)std::max<long long>(1, std::min<long long>(nb, (256ll << 20) / (8 * row_words)));
-    DevAlloc rel_dev(sizeof(u64) * row_words * slice);
+    const long long bins_bytes = Arena::round(4ll * nb);
+    PlaneStaging local;
+    PlaneStaging& st = staging ? *staging : local;
+    char* base = st.get(bins_bytes + 8ll * row_words * slice);
+    int* bins = reinterpret_cast<int*>(base);
+    u64* rel_dev = reinterpret_cast<u64*>(base + bins_bytes);
+    HIPC(hipMemcpyAsync(bins, im.bin_cons.data(), 4ll * nb, hipMemcpyHostToDevice, s));
     for (int k0 = 0; k0 < nb; k0 += slice) {
       const int k1 = std::min(nb, k0 + slice);
-      HIPC(hipMemcpyAsync(rel_dev.p, im.rel + (long long)k0 * row_words,
+      HIPC(hipMemcpyAsync(rel_dev, im.rel + (long long)k0 * row_words,
                           sizeof(u64) * row_words * (k1 - k0), hipMemcpyHostToDevice, s));
       const long long waves = (long long)(k1 - k0) * im.W;
       const int threads = 256;
       const long long blocks = (waves * 64 + threads - 1) / threads;
       hipLaunchKernelGGL(k_build_planes, dim3((unsigned)blocks), dim3(threads), 0, s,
-                         rel_dev.as<u64>(), k1 - k0, im.max_dom, im.rel_words, im.W, im.dpad,
-                         bins.as<int>() + k0, dm.con.as<ConMeta>(), dm.cap.as<int>(), dm.sup.as<u64>());
+                         rel_dev, k1 - k0, im.max_dom, im.rel_words, im.W, im.dpad,
+                         bins + k0, dm.con.as<ConMeta>(), dm.cap.as<int>(), dm.sup.as<u64>());
       HIPC(hipGetLastError());
-      HIPC(hipStreamSynchronize(s));
+      if (k1 < nb) HIPC(hipStreamSynchronize(s));  // the next slice overwrites rel_dev
     }
   }
   std::vector<char> lds_image;  // host side of the image copy, until the sync below
-  build_lds_image(im, dm, s, lds_image);
+  build_lds_image(im, dm, s, lds_image, arena);
   HIPC(hipStreamSynchronize(s));
 }
 
@@ -1247,6 +1335,7 @@ Knob g_knobs[] = {
     {"sac_slice_nibbles", "CPIM_SLICE_NIBBLES", {0}, false},
     {"mac_lds_net", "CPIM_MAC_LDS_NET", {0}, false},
     {"fixpoint_budget_us", "CPIM_FIXPOINT_BUDGET_US", {0}, false},
+    {"sac_slice_bytes", "CPIM_SLICE_BYTES", {0}, false},
 };
 std::mutex g_knob_mu;
 
@@ -1255,6 +1344,7 @@ bool knob_valid(const char* name, long long v) {
   if (k == "sac_slice_masks") return v == 0 || v == 1 || v == 2 || v == 4;
   if (k == "sac_slice_kernel") return v == 0 || v == 1;
   if (k == "sac_slice_nibbles") return v == 0 || v == 1 || v == 2;
+  if (k == "sac_slice_bytes") return v == 0 || v == 1 || v == 4 || v == 6;
   if (k == "mac_lds_net") return v == 0 || v == 1;
   if (k == "fixpoint_budget_us") return v >= 0 && v <= 600000000ll;
   return v == 0 || (v >= 64 && v <= 1024 && v % 64 == 0);
@@ -1320,6 +1410,17 @@ bool use_nibble_tables(const cpim_gmodel* m, int M) {
   return b <= (k == 1 ? (long long)m->props.max_lds : std::min<long long>(m->props.max_lds, 80 * 1024));
 }
 
+// Byte tables in front of the nibble tables (sac_slice_bytes: 4 or 6 = that
+// many whenever the single stage buffer fits; 0 and 1 = none).
+int byte_tables_for(const cpim_gmodel* m, int M, bool nib) {
+  const int k = knob("sac_slice_bytes");
+  if (!nib || k <= 1) return 0;
+  const long long limit = k ? (long long)m->props.max_lds : std::min<long long>(m->props.max_lds, 80 * 1024);
+  for (int nb : {6, 4})
+    if ((k == 0 || k == nb) && wide_lds_bytes(m->dm.n, m->dm.W, M, true, nb) <= limit) return nb;
+  return 0;
+}
+
 bool sliced_eligible(const cpim_gmodel* m) {
   if ((int)m->im.bin_cons.size() != m->im.num_active) return false;
   return use_wide_kernel() ? wide_masks_for(m, 1) > 0 : slice_threads_for(m) > 0;
@@ -1348,6 +1449,7 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
   const bool wide = use_wide_kernel();
   const int M = wide ? wide_masks_for(m, count) : 1;
   const bool nib = wide && use_nibble_tables(m, M);
+  const int nb = wide ? byte_tables_for(m, M, nib) : 0;
   REQUIRE(M > 0, "bit-sliced singleton tests do not fit in LDS");
   const long long cap = 64ll * M;  // tests per workgroup
   std::vector<int> gfirst;
@@ -1471,7 +1573,7 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
     // most the kernel's launch bound
     threads = knob("sac_slice_threads") ? knob("sac_slice_threads") : (M >= 2 ? 1024 : 512);
     threads = std::min(std::max(threads, 64 * M), kWideMaxThreads);
-    lds = (size_t)wide_lds_bytes(m->dm.n, m->dm.W, M, nib);
+    lds = (size_t)wide_lds_bytes(m->dm.n, m->dm.W, M, nib, nb);
   } else {
     threads = slice_threads_for(m);
     lds = (size_t)slice_lds_bytes(m->dm.n, m->dm.dpad, threads / 64);
@@ -1487,11 +1589,19 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
     unsigned char* fl = S.fails.as<unsigned char>();
     unsigned long long* rv = S.revs.as<unsigned long long>();
     if (!wide) hipLaunchKernelGGL(k_sliced_tests, grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices);
-#define CPIM_WIDE(MM, NB) \
-  hipLaunchKernelGGL((k_sliced_wide<MM, NB>), grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices)
-    else if (M == 1) { if (nib) CPIM_WIDE(1, true); else CPIM_WIDE(1, false); }
-    else if (M == 2) { if (nib) CPIM_WIDE(2, true); else CPIM_WIDE(2, false); }
-    else { if (nib) CPIM_WIDE(4, true); else CPIM_WIDE(4, false); }
+#define CPIM_WIDE(MM, NIB, NB) \
+  hipLaunchKernelGGL((k_sliced_wide<MM, NIB, NB>), grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices)
+#define CPIM_WIDE_M(MM)                                \
+  {                                                    \
+    if (nb == 6) CPIM_WIDE(MM, true, 6);               \
+    else if (nb == 4) CPIM_WIDE(MM, true, 4);          \
+    else if (nib) CPIM_WIDE(MM, true, 0);              \
+    else CPIM_WIDE(MM, false, 0);                      \
+  }
+    else if (M == 1) CPIM_WIDE_M(1)
+    else if (M == 2) CPIM_WIDE_M(2)
+    else CPIM_WIDE_M(4)
+#undef CPIM_WIDE_M
 #undef CPIM_WIDE
     HIPC(hipGetLastError());
   }

@@ -144,10 +144,12 @@ __device__ __forceinline__ void grid_process_constraint(const DevNet& net, u64* 
 // returned value.
 constexpr int kPairBatch = 4;
 
+// The revisions of one batch; returns the mask of lanes whose variable (in
+// myv) lost values, the values removed in the batch, and whether some domain
+// was emptied by it.
 template <bool kCoherent>
-__device__ __forceinline__ void grid_pairs_w1(const DevNet& net, u64* dom, int* size, const int (&cs)[kPairBatch],
-                                              int k, unsigned* B, GacCtl* ctl, int r,
-                                              unsigned long long& revs, unsigned long long& dels) {
+__device__ __forceinline__ u64 revise_pairs_w1(const DevNet& net, u64* dom, int* size, const int (&cs)[kPairBatch],
+                                               int k, int& myv, int& removed_total, bool& wiped_any) {
   const int lane = lane_id();
   int xs[kPairBatch], ys[kPairBatch];
   long long offs[kPairBatch];
@@ -159,7 +161,7 @@ __device__ __forceinline__ void grid_pairs_w1(const DevNet& net, u64* dom, int* 
     offs[p] = mp->off;
   }
   // lane 2p: D(x_p), lane 2p + 1: D(y_p)
-  int myv = -1;
+  myv = -1;
 #pragma unroll
   for (int p = 0; p < kPairBatch; ++p) {
     if (p < k && lane == 2 * p) myv = xs[p];
@@ -183,7 +185,6 @@ __device__ __forceinline__ void grid_pairs_w1(const DevNet& net, u64* dom, int* 
     if (lane == 2 * p) keep_l = kx;
     if (lane == 2 * p + 1) keep_l = ky;
   }
-  revs += 2 * k;
   int removed_l = 0;
   bool wiped_l = false;
   if (myv >= 0 && (dl & ~keep_l)) {
@@ -192,10 +193,21 @@ __device__ __forceinline__ void grid_pairs_w1(const DevNet& net, u64* dom, int* 
     wiped_l = old != 0ull && (old & keep_l) == 0ull;
     if (removed_l) atomicSub(&size[myv], removed_l);
   }
-  if (__ballot(wiped_l) && lane == 0) atomicMin(&ctl->wipe_round, r);
-  u64 changed = __ballot(removed_l > 0);
-  const int total = (int)wave_sum64_dpp((u64)removed_l);
-  dels += total;
+  wiped_any = __ballot(wiped_l) != 0ull;
+  removed_total = (int)wave_sum64_dpp((u64)removed_l);
+  return __ballot(removed_l > 0);
+}
+
+template <bool kCoherent>
+__device__ __forceinline__ void grid_pairs_w1(const DevNet& net, u64* dom, int* size, const int (&cs)[kPairBatch],
+                                              int k, unsigned* B, GacCtl* ctl, int r,
+                                              unsigned long long& revs, unsigned long long& dels) {
+  int myv = -1, removed = 0;
+  bool wiped = false;
+  u64 changed = revise_pairs_w1<kCoherent>(net, dom, size, cs, k, myv, removed, wiped);
+  revs += 2 * k;
+  dels += removed;
+  if (wiped && lane_id() == 0) atomicMin(&ctl->wipe_round, r);
   while (changed) {
     const int l = (int)__builtin_ctzll(changed);
     changed &= changed - 1ull;
@@ -508,10 +520,45 @@ __global__ void k_gac_async(DevNet net, u64* dom, int* size, unsigned* F, int nf
       claimed = __shfl(claimed, 0, 64);
       const int nclaimed = __popc(claimed);
       const int base_c = (chunk * 64 + l) * 32;
-      while (claimed) {
-        const int b = __ffs(claimed) - 1;
-        claimed &= claimed - 1u;
-        async_process_constraint(net, dom, size, base_c + b, F, ctl, revs, dels);
+      if (net.W == 1) {
+        // one-word domains: binary constraints kPairBatch at a time
+        claimed = (unsigned)uniform_int((int)claimed);
+        while (claimed) {
+          int cs[kPairBatch] = {0, 0, 0, 0};
+          int k = 0;
+          while (claimed && k < kPairBatch) {
+            const int b = __ffs(claimed) - 1;
+            claimed &= claimed - 1u;
+            const int c = base_c + b;
+            if (net.con[c].kind != 0) {
+              async_process_constraint(net, dom, size, c, F, ctl, revs, dels);
+              continue;
+            }
+#pragma unroll
+            for (int p = 0; p < kPairBatch; ++p)
+              if (p == k) cs[p] = c;
+            ++k;
+          }
+          if (k) {
+            int myv = -1, removed = 0;
+            bool wiped = false;
+            u64 changed = revise_pairs_w1<true>(net, dom, size, cs, k, myv, removed, wiped);
+            revs += 2 * k;
+            dels += removed;
+            if (wiped && lane == 0) atomicExch(&ctl->wipe, 1);
+            while (changed) {
+              const int cl = (int)__builtin_ctzll(changed);
+              changed &= changed - 1ull;
+              push_var_async(net, __builtin_amdgcn_readlane(myv, cl), F, ctl);
+            }
+          }
+        }
+      } else {
+        while (claimed) {
+          const int b = __ffs(claimed) - 1;
+          claimed &= claimed - 1u;
+          async_process_constraint(net, dom, size, base_c + b, F, ctl, revs, dels);
+        }
       }
       if (nclaimed && lane == 0) atomicSub(&ctl->pending, (unsigned long long)nclaimed);
       seen = 0;
@@ -1085,7 +1132,7 @@ __device__ __forceinline__ bool slice_revise(const u64* __restrict__ plane_u, u6
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int j = j0 + q;
-        need[q] = j < W ? ld_dom<true>(&slu[j * 64 + lane]) : 0ull;
+        need[q] = j < W ? ld_wg(&slu[j * 64 + lane]) : 0ull;
       }
     } else {
 #pragma unroll
@@ -1115,7 +1162,7 @@ __device__ __forceinline__ bool slice_revise(const u64* __restrict__ plane_u, u6
         pend = clear_supported(pend, (unsigned)(r >> 32), st + 34);
       }
       if (pend) {
-        const u64 old = atomicAnd(&slu[b], ~pend);
+        const u64 old = and_wg(&slu[b], ~pend);
         if (old & pend) changed = true;
       }
     }
@@ -1173,15 +1220,14 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
     ctl.act = all;
     ctl.revs = 0;
   }
-  __threadfence();
   __syncthreads();
   // Test k reduces its variable x_k to {a_k}.
   for (int k = wave; k < tc; k += nwaves) {
     const int2 tk = tests[tf + k];
     for (int b = lane; b < dpad; b += 64) {
       u64* p = &sl[(long long)tk.x * dpad + b];
-      if (b == tk.y) atomicOr(p, 1ull << k);
-      else atomicAnd(p, ~(1ull << k));
+      if (b == tk.y) or_wg(p, 1ull << k);
+      else and_wg(p, ~(1ull << k));
     }
   }
   if (tid < tc) {
@@ -1189,7 +1235,6 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
     atomicOr(&cur[x >> 5], 1u << (x & 31));
     atomicOr(&touched[x >> 5], 1u << (x & 31));
   }
-  __threadfence();
   __syncthreads();
 
   unsigned long long revs = 0;
@@ -1256,7 +1301,7 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
       // stage the slices of t and the values alive in some live test
       const u64* slt = sl + (long long)t * dpad;
       for (int i = lane; i < dpad; i += 64) {
-        const u64 v = ld_dom<true>(&slt[i]);
+        const u64 v = ld_wg(&slt[i]);
         stage[stage_pos(i)] = v;
         const u64 alive = __ballot((v & act) != 0ull);
         if (lane == 0) atw[i >> 6] = alive;
@@ -1277,14 +1322,13 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
       }
       __builtin_amdgcn_wave_barrier();
     }
-    __threadfence();
     __syncthreads();
     // Exact wipe-out check: a test wiped out at v only if v changed.
     u64 alive_all = ~0ull;
     for (int v = wave; v < n; v += nwaves) {
       if (!uniform_int((int)bit_of(nxt, v))) continue;
       u64 o = 0ull;
-      for (int i = lane; i < dpad; i += 64) o |= ld_dom<true>(&sl[(long long)v * dpad + i]);
+      for (int i = lane; i < dpad; i += 64) o |= ld_wg(&sl[(long long)v * dpad + i]);
       for (int d = 32; d > 0; d >>= 1) o |= __shfl_xor(o, d, 64);
       alive_all &= o;
     }
@@ -1336,18 +1380,25 @@ struct WideCtl {
 };
 
 // Words of one stage buffer: kStageWord * W entries of M masks, or with
-// nibble tables (nib) 16 W nibbles x 16 subsets of M masks.
-__host__ __device__ inline long long wide_stage_words(int W, int M, bool nib) {
-  return nib ? 256ll * W * M : (long long)kStageWord * W * M;
+// nibble tables (nib) 16 W nibbles x 16 subsets of M masks.  With nb byte
+// tables the first nb bytes of word 0 have one 256-subset table each in
+// place of their 2 nb nibble tables.
+__host__ __device__ inline long long wide_stage_words(int W, int M, bool nib, int nb = 0) {
+  if (!nib) return (long long)kStageWord * W * M;
+  return 256ll * nb * M + 16ll * (16 * W - 2 * nb) * M;
 }
 
-// LDS bytes: list [n], cur/nxt/touched bitmaps, two stage buffers, two
+// Stage buffers: two (double buffered) without byte tables, one with them.
+__host__ __device__ inline int wide_stage_buffers(int nb) { return nb > 0 ? 1 : 2; }
+
+// LDS bytes: list [n], cur/nxt/touched bitmaps, the stage buffers and as many
 // alive-value rows of W words.
-__host__ __device__ inline long long wide_lds_bytes(int n, int W, int M, bool nib = false) {
+__host__ __device__ inline long long wide_lds_bytes(int n, int W, int M, bool nib = false, int nb = 0) {
   const long long nvw = (n + 31) / 32;
   long long b = 4ll * n + 12ll * nvw;
   b = (b + 15) & ~15ll;
-  b += 2ll * 8 * wide_stage_words(W, M, nib) + 2ll * 8 * W;
+  const int nbuf = wide_stage_buffers(nb);
+  b += nbuf * 8ll * wide_stage_words(W, M, nib, nb) + nbuf * 8ll * W;
   return b;
 }
 
@@ -1407,9 +1458,9 @@ __device__ __forceinline__ int nib_word(int q, int sub, int mm) {
   else return (q * 16 + sub) * M + mm;
 }
 
-template <int M>
+template <int M, int Q0 = 0>
 __device__ __forceinline__ void clear_supported_nib(u64 (&pend)[M], unsigned bits, const u64* tab) {
-  for (int q = 0; q < 8 && bits && any_mask<M>(pend); q += kNibStep) {
+  for (int q = Q0; q < 8 && bits && any_mask<M>(pend); q += kNibStep) {
     u64 acc[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) acc[m] = 0ull;
@@ -1426,7 +1477,58 @@ __device__ __forceinline__ void clear_supported_nib(u64 (&pend)[M], unsigned bit
   }
 }
 
-template <int M, bool kNib>
+// Byte tables: entry (p, s) of byte p of word 0 holds the OR of the masks of
+// the values 8p + i for the set bits i of s, so that one lookup covers 8
+// values where a nibble table covers 4.  A support is usually found among the
+// first few dozen values of a row (at tightness 0.5 a test keeps about half
+// the values of each changed neighbour), so the tables cover the start of
+// word 0 only; nibble tables cover the rest.  With four masks an entry is
+// split in two 16-byte halves in separate planes, as in the nibble tables.
+template <int M>
+__device__ __forceinline__ int byte_word(int p, int sub, int mm) {
+  if constexpr (M == 4) return p * 1024 + (mm >> 1) * 512 + sub * 2 + (mm & 1);
+  else return (p * 256 + sub) * M + mm;
+}
+
+// Byte lookups per step: 3 for 6 tables, else 2.
+__host__ __device__ constexpr int byte_step(int nb) { return nb % 3 == 0 ? 3 : 2; }
+
+// pend[m] &= ~(masks of the supports among values 0 .. 8 NB - 1 of word 0);
+// returns the row bits above them.
+template <int M, int NB>
+__device__ __forceinline__ u64 clear_supported_bytes(u64 (&pend)[M], u64 r, const u64* btab) {
+  constexpr int S = byte_step(NB);
+#pragma unroll
+  for (int p = 0; p < NB; p += S) {
+    if (!(r >> (8 * p)) || !any_mask<M>(pend)) break;
+    u64 acc[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) acc[m] = 0ull;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      if (p + i < NB) {
+        const int sub = (int)((r >> (8 * (p + i))) & 255ull);
+        const u64* e = btab + byte_word<M>(p + i, sub, 0);
+#pragma unroll
+        for (int m = 0; m < M; ++m) acc[m] |= e[byte_word<M>(0, 0, m)];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m) pend[m] &= ~acc[m];
+  }
+  return r & ~((1ull << (8 * NB)) - 1ull);
+}
+
+// Nibble tables of a stage buffer with NB byte tables in front: nibble q of
+// word 0 (q >= 2 NB) and every nibble of the later words sit where the
+// nibble-only layout would put them, shifted back by the LDS the byte tables
+// do not use.
+template <int M, int NB>
+__device__ __forceinline__ const u64* nib_base(const u64* stage) {
+  return stage + (256ll * NB - 32ll * NB) * M;
+}
+
+template <int M, bool kNib, int NB>
 __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64* slu, const u64* stage,
                                             const u64* atw, int W, int dpad, const u64 (&act)[M],
                                             bool touched_u, const u64* __restrict__ base_u,
@@ -1441,15 +1543,20 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
     if (touched_u) {
 #pragma unroll
       for (int m = 0; m < M; ++m)
-        pend[m] = all[m] & act[m] & (ld_dom<true>(&slu[(long long)b * M + m]) ^ flip);
+        pend[m] = all[m] & act[m] & (ld_wg(&slu[(long long)b * M + m]) ^ flip);
     } else {
 #pragma unroll
       for (int m = 0; m < M; ++m) pend[m] = in ? (all[m] & act[m]) : 0ull;
     }
     const u64 row0 = plane_u[b];
+    const u64* nib = nib_base<M, NB>(stage);
     if (any_mask<M>(pend)) {
       const u64 r = row0 & atw[0];
-      if (kNib) {
+      if (NB > 0) {
+        const u64 rest = clear_supported_bytes<M, NB>(pend, r, stage);
+        if (NB < 4) clear_supported_nib<M, 2 * NB>(pend, (unsigned)rest, nib);
+        clear_supported_nib<M, (NB > 4 ? 2 * NB - 8 : 0)>(pend, (unsigned)(rest >> 32), nib + 128ll * M);
+      } else if (kNib) {
         clear_supported_nib<M>(pend, (unsigned)r, stage);
         clear_supported_nib<M>(pend, (unsigned)(r >> 32), stage + 128ll * M);
       } else {
@@ -1460,7 +1567,7 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
     for (int w = 1; w < W && any_mask<M>(pend); ++w) {
       const u64 r = plane_u[(long long)w * dpad + b] & atw[w];
       if (kNib) {
-        const u64* tb = stage + 256ll * w * M;
+        const u64* tb = nib + 256ll * w * M;
         clear_supported_nib<M>(pend, (unsigned)r, tb);
         clear_supported_nib<M>(pend, (unsigned)(r >> 32), tb + 128ll * M);
       } else {
@@ -1476,10 +1583,10 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
           // removal: set the bit of a base value, clear that of a value a
           // test made present
           if (in) {
-            const u64 old = atomicOr(&slu[(long long)b * M + m], pend[m]);
+            const u64 old = or_wg(&slu[(long long)b * M + m], pend[m]);
             if (pend[m] & ~old) changed = true;
           } else {
-            const u64 old = atomicAnd(&slu[(long long)b * M + m], ~pend[m]);
+            const u64 old = and_wg(&slu[(long long)b * M + m], ~pend[m]);
             if (pend[m] & old) changed = true;
           }
         }
@@ -1497,7 +1604,7 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
 constexpr int kWideMaxThreads = 1024;
 constexpr int kWideWavesPerSimd = 8;
 
-template <int M, bool kNib>
+template <int M, bool kNib, int NB>
 __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_wide(DevNet net, const u64* __restrict__ base_dom,
                               const int2* __restrict__ tests, const int* __restrict__ gfirst,
                               long long g0, const int* __restrict__ perm, unsigned char* fail,
@@ -1528,8 +1635,9 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
   unsigned* touched = nxt + nvw;
   const long long head = (4ll * n + 12ll * nvw + 15) & ~15ll;
   u64* stage0 = (u64*)(smem + head);
-  const long long stage_words = wide_stage_words(W, M, kNib);
-  const int nbuf = 2;  // double buffer
+  static_assert(NB == 0 || (kNib && NB <= 6), "byte tables go with nibble tables");
+  const long long stage_words = wide_stage_words(W, M, kNib, NB);
+  constexpr int nbuf = NB > 0 ? 1 : 2;  // double buffer without byte tables
   u64* atw0 = stage0 + nbuf * stage_words;
 
   for (int i = tid; i < nvw; i += nthreads) {
@@ -1548,7 +1656,6 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
     for (int m = 0; m < M; ++m) ctl.act[m] = all[m];
     ctl.revs = 0;
   }
-  __threadfence();
   __syncthreads();
   // test k reduces its variable x_k to {a_k}: flip every base value but a_k,
   // and a_k if it is not a base value
@@ -1558,7 +1665,7 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
     const u64 bit = 1ull << (k & 63);
     for (int b = lane; b < dpad; b += 64) {
       const bool in = (base_dom[(long long)tk.x * W + (b >> 6)] >> (b & 63)) & 1ull;
-      if (in != (b == tk.y)) atomicOr(&sl[((long long)tk.x * dpad + b) * M + m], bit);
+      if (in != (b == tk.y)) or_wg(&sl[((long long)tk.x * dpad + b) * M + m], bit);
     }
   }
   if (tid < tc) {
@@ -1566,7 +1673,6 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
     atomicOr(&cur[x >> 5], 1u << (x & 31));
     atomicOr(&touched[x >> 5], 1u << (x & 31));
   }
-  __threadfence();
   __syncthreads();
 
   unsigned long long revs = 0;
@@ -1597,8 +1703,10 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
     for (int m = 0; m < M; ++m) act[m] = (u64)uniform_ll((long long)ctl.act[m]);
     for (int i = 0; i < mcount; ++i) {
       const int t = uniform_int(list[i]);
-      u64* stage = stage0 + (i & 1) * stage_words;
-      u64* atw = atw0 + (i & 1) * W;
+      // one buffer: every wave finishes the previous variable's arcs first
+      if (nbuf == 1 && i > 0) __syncthreads();
+      u64* stage = stage0 + (i & (nbuf - 1)) * stage_words;
+      u64* atw = atw0 + (i & (nbuf - 1)) * W;
       // stage t: thread v holds value v's M masks; one wave covers one word
       const u64* slt = sl + (long long)t * dpad * M;
       const u64* base_t = base_dom + (long long)t * W;
@@ -1607,25 +1715,49 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
         const u64 flip = ((base_t[v >> 6] >> (v & 63)) & 1ull) ? ~0ull : 0ull;
         const int kk = tc - 64 * mm;
         const u64 all_m = kk >= 64 ? ~0ull : (kk > 0 ? ((1ull << kk) - 1ull) : 0ull);
-        return all_m & (ld_dom<true>(&slt[(long long)v * M + mm]) ^ flip);
+        return all_m & (ld_wg(&slt[(long long)v * M + mm]) ^ flip);
       };
-      // every wave finished the arcs of buffer (i & 1)'s previous variable
+      // every wave finished the arcs of this buffer's previous variable
       // before the last barrier
       if (tid == 0) ctl.arc_ctr[i & 1] = 0;
       if (kNib) {
         // nibble tables: item (q, m) ORs the 16 subsets of values 4q..4q+3
-        for (int it = tid; it < 16 * W * M; it += nthreads) {
-          const int q = it / M, mm = it - q * M;
-          u64 x[4];
+        // (nibbles from 2 NB on); byte tables: item (p, m, h) fills the 16
+        // subsets of byte p whose high nibble is h
+        u64* nib = stage + (256ll * NB - 32ll * NB) * M;
+        const int nib_items = (16 * W - 2 * NB) * M;
+        for (int it = tid; it < nib_items + NB * M * 16; it += nthreads) {
+          if (it < nib_items) {
+            const int q = 2 * NB + it / M, mm = it % M;
+            u64 x[4];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) x[i] = alive_of(4 * q + i, mm);
+            for (int i = 0; i < 4; ++i) x[i] = alive_of(4 * q + i, mm);
 #pragma unroll
-          for (int sub = 0; sub < 16; ++sub) {
-            u64 o = 0ull;
+            for (int sub = 0; sub < 16; ++sub) {
+              u64 o = 0ull;
+#pragma unroll
+              for (int i = 0; i < 4; ++i)
+                if (sub & (1 << i)) o |= x[i];
+              nib[nib_word<M>(q, sub, mm)] = o;
+            }
+          } else {
+            const int k = it - nib_items;
+            const int h = k & 15, mm = (k >> 4) % M, p = (k >> 4) / M;
+            u64 x[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) x[i] = alive_of(8 * p + i, mm);
+            u64 hi = 0ull;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-              if (sub & (1 << i)) o |= x[i];
-            stage[nib_word<M>(q, sub, mm)] = o;
+              if (h & (1 << i)) hi |= x[4 + i];
+#pragma unroll
+            for (int lo = 0; lo < 16; ++lo) {
+              u64 o = hi;
+#pragma unroll
+              for (int i = 0; i < 4; ++i)
+                if (lo & (1 << i)) o |= x[i];
+              stage[byte_word<M>(p, (h << 4) | lo, mm)] = o;
+            }
           }
         }
       }
@@ -1655,7 +1787,7 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
         const long long o = (long long)uniform_int(e.w) * 2 * W * dpad;
         const u64* plane_u = net.sup + o + (long long)(1 - pos_t) * W * dpad;
         const bool tu = uniform_int((int)bit_of(touched, u));
-        const bool ch = wide_revise<M, kNib>(plane_u, sl + (long long)u * dpad * M, stage, atw, W, dpad, act,
+        const bool ch = wide_revise<M, kNib, NB>(plane_u, sl + (long long)u * dpad * M, stage, atw, W, dpad, act,
                                              tu, base_dom + (long long)u * W, all);
         ++revs;
         if (ch && lane == 0) {
@@ -1665,7 +1797,6 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
         }
       }
     }
-    __threadfence();
     __syncthreads();
     // exact wipe-out check: a test wiped out at v only if v changed
     u64 alive_all[M];
@@ -1678,7 +1809,7 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
         u64 o = 0ull;
         for (int b = lane; b < dpad; b += 64) {
           const u64 flip = ((base_dom[(long long)v * W + (b >> 6)] >> (b & 63)) & 1ull) ? ~0ull : 0ull;
-          o |= all[m] & (ld_dom<true>(&sl[((long long)v * dpad + b) * M + m]) ^ flip);
+          o |= all[m] & (ld_wg(&sl[((long long)v * dpad + b) * M + m]) ^ flip);
         }
         o = wave_or64_dpp(o);
         alive_all[m] &= o;

@@ -618,7 +618,7 @@ class Server:
     def result(self, rid: int):
         """(done, status, num_positive, num_negative, solution or None, steps)."""
         r = N.CRequestResult()
-        sol = np.full(self._n[rid], -1, np.int32)
+        sol = np.full(max(1, self._n.get(rid, 0)), -1, np.int32)  # unknown ids fail in the call
         code = N.gpu_lib().cpim_server_result(self._h, rid, C.byref(r), sol.ctypes.data_as(N.p_i32))
         if code == N.CPIM_ERR_BUSY:
             return None
@@ -626,3 +626,15 @@ class Server:
         return dict(status=r.status, num_positive=r.num_positive, num_negative=r.num_negative,
                     solution=sol if r.status == 1 else None, steps=r.steps,
                     footprint_bytes=r.footprint_bytes)
+
+    def release(self, rid: int) -> None:
+        """Drop a finished request's result (the server keeps each one until then)."""
+        N.check(N.gpu_lib().cpim_server_release(self._h, rid))
+        self._n.pop(rid, None)
+
+    def memory(self) -> dict:
+        """Device bytes charged to resident requests (now and peak), queued and resident counts."""
+        cur, peak = C.c_int64(0), C.c_int64(0)
+        q, r = C.c_int32(0), C.c_int32(0)
+        N.check(N.gpu_lib().cpim_server_memory(self._h, C.byref(cur), C.byref(peak), C.byref(q), C.byref(r)))
+        return dict(resident_bytes=cur.value, peak_bytes=peak.value, queued=q.value, resident=r.value)

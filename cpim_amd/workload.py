@@ -143,6 +143,35 @@ def uneven_instance(index: int) -> ModelDesc:
                          name=f"c4-{index}")
 
 
+def serving_request_c3(index: int, n: int = 500, d: int = 256, e: int = 20000, p: float = 0.5,
+                       n_tables: int = 4, table_tuples: int = 500, table_dom: int = 16) -> ModelDesc:
+    """A serving request of the benchmark SAC network's shape: a random binary
+    CSP n=500, d=256, e=20000, t=0.5 (its own seed per request) plus n_tables
+    arity-3 positive tables over distinct random triples of its variables,
+    each with table_tuples random distinct tuples over the values
+    0..table_dom-1 (the tables therefore restrict their variables to those 16
+    values)."""
+    s = instance_seed(6, index)
+    base = random_binary(n, d, e, p, s ^ 0xC, name=f"c5c3-{index}")
+    triples, tables = [], []
+    for t in range(n_tables):
+        a = uniform_int(s, 100 + 3 * t, 0, n - 1)
+        b = uniform_int(s, 101 + 3 * t, 0, n - 2)
+        b = b + 1 if b >= a else b
+        c = uniform_int(s, 102 + 3 * t, 0, n - 3)
+        lo, hi = sorted((a, b))
+        c = c + 1 if c >= lo else c
+        c = c + 1 if c >= hi else c
+        triples.append(np.array([a, b, c], np.int32))
+        tables.append(random_table(s ^ (0x7000 + t), 3, table_dom, table_tuples))
+    arity = np.concatenate([base.con_arity, np.full(n_tables, 3, np.int32)])
+    scope = np.concatenate([base.con_scope] + triples).astype(np.int32)
+    counts = [0] * e + [t.size for t in tables]
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    flat = np.concatenate([t.reshape(-1) for t in tables]).astype(np.int32)
+    return ModelDesc(base.dom_size, arity, scope, base.bin_rel, off, flat, None, f"c5c3-{index}")
+
+
 def serving_request(index: int, n_lo: int = 20, n_hi: int = 40, d: int = 16,
                     density: float = 0.25, n_tables: int = 4, table_tuples: int = 500,
                     tight_scale: float = 0.85) -> ModelDesc:

@@ -224,6 +224,9 @@ typedef struct cpim_sac_stats {
  *   "sac_slice_kernel"  0 = one shared stage per changed variable, 1 = one stage per wave
  *   "sac_slice_nibbles" nibble (four-Russians) tables in the shared stage: 0 = when they fit
  *                       in 80 KB of LDS, 1 = whenever they fit, 2 = never
+ *   "sac_slice_bytes"   256-subset tables for the first bytes of each staged row in front of
+ *                       the nibble tables (one stage buffer instead of two): 4 or 6 = that
+ *                       many whenever they fit, 0 and 1 = none
  *   "unit_threads"      threads per workgroup of the batch / serving MAC kernel
  *   "mac_lds_net"       device MAC: 0 = copy the network (adjacency, planes, tables) into LDS
  *                       when it fits, 1 = read it from device memory
@@ -231,7 +234,8 @@ typedef struct cpim_sac_stats {
  *                       launch in microseconds (0 = 20 s); a launch that exceeds it stops and
  *                       the call returns CPIM_ERR_RUNTIME
  * First values come from CPIM_SLICE_MASKS, CPIM_SLICE_THREADS, CPIM_SLICE_KERNEL,
- * CPIM_SLICE_NIBBLES, CPIM_UNIT_THREADS, CPIM_MAC_LDS_NET, CPIM_FIXPOINT_BUDGET_US. */
+ * CPIM_SLICE_NIBBLES, CPIM_UNIT_THREADS, CPIM_MAC_LDS_NET, CPIM_FIXPOINT_BUDGET_US,
+ * CPIM_SLICE_BYTES. */
 int cpim_tune_set(const char* name, int64_t value);
 int cpim_tune_get(const char* name, int64_t* value);
 
@@ -383,9 +387,13 @@ int cpim_server_destroy(cpim_server* server);
 /* Queue a request; the model is copied, so desc may be freed on return. */
 int cpim_server_submit(cpim_server* server, const cpim_model_desc* desc, int64_t node_limit,
                        int64_t* request_id);
-/* Admit queued requests into free slots (FIFO, by HBM footprint), advance
- * every resident request by nodes_per_step nodes in one launch, retire the
- * finished ones.  *active receives the number still resident or queued. */
+/* Advance every resident request by nodes_per_step nodes in one launch and
+ * retire the finished ones.  While that launch runs, queued requests are
+ * admitted (FIFO, while a slot and hbm_budget_bytes allow; a request that
+ * finds the device empty is admitted whatever its size) on a second stream;
+ * they join the next step.  A request is charged exactly the device bytes it
+ * takes: one allocation holding its network and search state.  *active
+ * receives the number still resident or queued. */
 int cpim_server_step(cpim_server* server, int32_t* active);
 /* Ids of the requests finished since the last poll, oldest first (at most
  * cap; the rest stay for the next poll).  No counterpart in the reference. */
@@ -394,6 +402,14 @@ int cpim_server_poll(cpim_server* server, int64_t* ids, int32_t cap, int32_t* co
  * CPIM_ERR_BUSY while the request is not finished. */
 int cpim_server_result(cpim_server* server, int64_t request_id, cpim_request_result* out,
                        int32_t* solution);
+/* Drop a finished request's result (its solution and record); the server
+ * keeps every finished result until it is released.  CPIM_ERR_BUSY while the
+ * request is not finished. */
+int cpim_server_release(cpim_server* server, int64_t request_id);
+/* Device bytes charged to resident requests now and at most so far, and the
+ * numbers of queued and resident requests (any pointer may be NULL). */
+int cpim_server_memory(const cpim_server* server, int64_t* resident_bytes, int64_t* peak_bytes,
+                       int32_t* queued, int32_t* resident);
 
 #ifdef __cplusplus
 }

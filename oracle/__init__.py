@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 AC3BIT = 0
 AC3 = 1
+AC3MIX = 2
 
 
 class OrcStats(C.Structure):

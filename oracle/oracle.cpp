@@ -119,11 +119,12 @@ std::shared_ptr<Shared> build_shared(const cpim_model_desc* d, int algo) {
     const uint64_t* row = d->bin_rel + ((size_t)bin_of[c] * S->max_dom + a) * Wr;
     return (row[b >> 6] >> (b & 63)) & 1ull;
   };
-  if (algo == ORC_AC3) {
+  if (algo == ORC_AC3 || algo == ORC_AC3MIX) {
     S->tuples.resize(S->ntabs);
     for (int c = 0; c < S->ntabs; ++c) {
       auto& T = S->tuples[c];
       const int ar = (int)S->scope[c].size();
+      if (algo == ORC_AC3MIX && ar == 2) continue;  // revised through bitSup rows
       if (dense(c)) {
         const int x = S->scope[c][0], y = S->scope[c][1];
         for (int a = 0; a < S->cap[x]; ++a)
@@ -135,7 +136,8 @@ std::shared_ptr<Shared> build_shared(const cpim_model_desc* d, int algo) {
       }
       std::sort(T.begin(), T.end());
     }
-  } else {
+  }
+  if (algo == ORC_AC3BIT || algo == ORC_AC3MIX) {
     // AC3bit: bitSup[(c, pos, a)][word] (AC3bit.cpp:10-26).
     S->bitsup.assign((size_t)S->ntabs * S->max_arity * S->max_dom * S->Wm, 0);
     auto set_bit = [&](size_t row, int b) { S->bitsup[row * S->Wm + (b >> 6)] |= 1ull << (b & 63); };
@@ -348,7 +350,9 @@ struct AC3 {
     }
   }
   bool seek(int c, int y, int a, int p) {
-    return N.S->algo == ORC_AC3BIT ? seek_bit(c, y, a, p) : seek_generic(c, y, a, p);
+    const int algo = N.S->algo;
+    if (algo == ORC_AC3BIT || (algo == ORC_AC3MIX && N.S->scope[c].size() == 2)) return seek_bit(c, y, a, p);
+    return seek_generic(c, y, a, p);
   }
   bool revise(int c, int y, int p) {
     ++revisions;

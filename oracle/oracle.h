@@ -41,6 +41,14 @@ typedef struct orc_net orc_net;
 
 #define ORC_AC3BIT 0  /* AC3 + AC3bit::seek_support (binary networks) */
 #define ORC_AC3 1     /* AC3 + generic tuple seek_support (any arity) */
+#define ORC_AC3MIX 2  /* AC3 + AC3bit::seek_support on binary constraints and the generic
+                         tuple search on the others.  A support is a support however it is
+                         found, so the closures, and the MAC nodes and constraint weights
+                         that follow them, are those of ORC_AC3; this mode exists because
+                         ORC_AC3 lists every allowed tuple of every binary relation, which
+                         does not fit in memory for 20,000 relations of 256 x 256 values.
+                         Not a reference combination: AC3bit refuses tables
+                         (src/AC3bit.cpp:4-6). */
 
 typedef struct orc_stats {
   int64_t num_positive;

@@ -143,3 +143,97 @@ def test_batch_with_a_unit_in_global_scratch(gpu):
             assert got.trace == r.trace, i
     finally:
         b.close()
+
+
+def drain(srv, ids, watch=None):
+    """Step until every request is done; watch(srv) runs after each step."""
+    finished, steps = [], 0
+    while srv.step() > 0:
+        finished += srv.poll()
+        if watch:
+            watch(srv)
+        steps += 1
+        assert steps < 100000
+    finished += srv.poll()
+    assert sorted(finished) == sorted(ids)
+
+
+def test_serving_hbm_budget_smaller_than_the_mix(gpu):
+    # the budget holds about two of these requests at once: the rest wait in
+    # the queue, the bytes charged never exceed the budget, and every result
+    # still equals the oracle's and the request solved alone
+    limit = 3000
+    reqs = [workload.serving_request(i) for i in range(10)]
+    probe = Server(max_slots=4, nodes_per_step=1)
+    rid = probe.submit(reqs[0], node_limit=1)
+    drain(probe, [rid])
+    one = probe.result(rid)["footprint_bytes"]
+    probe.close()
+    budget = int(2.5 * one)
+    srv = Server(max_slots=64, hbm_budget_bytes=budget, nodes_per_step=16)
+    seen = {"queued": 0, "resident": 0}
+
+    def watch(s):
+        mem = s.memory()
+        assert mem["resident_bytes"] <= budget and mem["peak_bytes"] <= budget
+        seen["queued"] = max(seen["queued"], mem["queued"])
+        seen["resident"] = max(seen["resident"], mem["resident"])
+
+    try:
+        ids = [srv.submit(r, node_limit=limit) for r in reqs]
+        drain(srv, ids, watch)
+        assert seen["queued"] > 0 and 1 <= seen["resident"] <= 3
+        for rid, desc in zip(ids, reqs):
+            got = srv.result(rid)
+            r = oracle.Oracle(desc, oracle.AC3).mac(node_limit=limit)
+            want = (oracle_status(r, limit), r.num_positive, r.num_negative,
+                    list(r.solution) if r.num_sol else None)
+            have = (got["status"], got["num_positive"], got["num_negative"],
+                    list(got["solution"]) if got["solution"] is not None else None)
+            assert have == want, rid
+            assert got["footprint_bytes"] <= budget
+            srv.release(rid)
+            with pytest.raises(Exception, match="unknown request id"):
+                srv.result(rid)
+        assert srv.memory()["resident_bytes"] == 0
+    finally:
+        srv.close()
+
+
+def test_serving_c3_shaped_requests_alone_and_in_the_mix(gpu):
+    # requests of the benchmark SAC network's shape (n=500, d=256, e=20000,
+    # t=0.5) plus four arity-3 tables of 500 tuples over 16 values, under a
+    # budget of 64 nodes: in a mix whose budget holds two at once, each result
+    # equals the request served alone and the oracle's (AC3 with bitset
+    # supports for the binary constraints, tuple search for the tables)
+    from concurrent.futures import ThreadPoolExecutor
+    limit = 64
+    reqs = [workload.serving_request_c3(i) for i in range(4)]
+
+    def oracle_run(desc):
+        r = oracle.Oracle(desc, oracle.AC3MIX).mac(node_limit=limit)
+        return (oracle_status(r, limit), r.num_positive, r.num_negative)
+
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        wants = list(ex.map(oracle_run, reqs))
+    alone = []
+    for desc in reqs:
+        s1 = Server(max_slots=1, nodes_per_step=32)
+        try:
+            rid = s1.submit(desc, node_limit=limit)
+            drain(s1, [rid])
+            g = s1.result(rid)
+            alone.append((g["status"], g["num_positive"], g["num_negative"]))
+            one = g["footprint_bytes"]
+        finally:
+            s1.close()
+    srv = Server(max_slots=8, hbm_budget_bytes=int(2.5 * one), nodes_per_step=16)
+    try:
+        ids = [srv.submit(r, node_limit=limit) for r in reqs]
+        drain(srv, ids)
+        for rid, want, a in zip(ids, wants, alone):
+            g = srv.result(rid)
+            assert (g["status"], g["num_positive"], g["num_negative"]) == want == a, rid
+        assert srv.memory()["peak_bytes"] <= int(2.5 * one)
+    finally:
+        srv.close()

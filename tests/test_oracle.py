@@ -684,3 +684,27 @@ def test_find_all_unsat_and_capacity():
     desc = workload.random_binary(8, 2, 4, 0.0, 3)
     r = oracle.mac_all(oracle.Oracle(desc), 10**6, sol_cap=10)
     assert r.num_sol == 2 ** 8 and r.solutions.shape == (10, 8)
+
+
+# ---------------------------------------------------------------------------
+# ORC_AC3MIX (bitset supports for binary constraints, tuple search for
+# tables) gives ORC_AC3's closures, so its SAC, MAC nodes and weights too.
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("seed", [5, 6, 7, 8])
+@pytest.mark.parametrize("varh", [0, 1, 2], ids=["dom", "domdeg", "domwdeg"])
+def test_ac3mix_equals_generic_ac3(seed, varh):
+    desc = workload.serving_request(seed, n_lo=10, n_hi=14, d=6, table_tuples=60)
+    a_ok, a_dom, _ = oracle.Oracle(desc, oracle.AC3).root_ac()
+    b_ok, b_dom, _ = oracle.Oracle(desc, oracle.AC3MIX).root_ac()
+    assert a_ok == b_ok and (not a_ok or (a_dom == b_dom).all())
+    a_ok, a_dom, _ = oracle.Oracle(desc, oracle.AC3).sac1()
+    b_ok, b_dom, _ = oracle.Oracle(desc, oracle.AC3MIX).sac1()
+    assert a_ok == b_ok and (not a_ok or (a_dom == b_dom).all())
+    with oracle.var_heuristic(varh):
+        oa = oracle.Oracle(desc, oracle.AC3)
+        ob = oracle.Oracle(desc, oracle.AC3MIX)
+        ra = oa.mac(node_limit=400, trace_cap=800)
+        rb = ob.mac(node_limit=400, trace_cap=800)
+    assert (ra.num_sol, ra.num_positive, ra.num_negative) == (rb.num_sol, rb.num_positive, rb.num_negative)
+    assert ra.trace == rb.trace
+    assert (oa.weights() == ob.weights()).all()
