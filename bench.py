@@ -51,13 +51,16 @@ def parse():
                     help="skip the single-fixpoint, uneven-batch and serving measurements")
     ap.add_argument("--batch-instances", type=int, default=4096,
                     help="instances of the 4096-instance uneven batch timed in the extras")
-    ap.add_argument("--serve-requests", type=int, default=256)
-    ap.add_argument("--serve-rate", type=float, default=2000.0,
+    ap.add_argument("--serve-requests", type=int, default=2048)
+    ap.add_argument("--serve-rate", type=float, default=1000.0,
                     help="Poisson arrival rate of serving requests (requests/s)")
-    ap.add_argument("--serve-c3-requests", type=int, default=48,
+    ap.add_argument("--serve-c3-requests", type=int, default=96,
                     help="requests of the benchmark SAC network's shape (with tables) served in the extras")
-    ap.add_argument("--serve-c3-rate", type=float, default=40.0,
+    ap.add_argument("--serve-c3-rate", type=float, default=8.0,
                     help="Poisson arrival rate of those requests (requests/s)")
+    ap.add_argument("--only", choices=["single_fixpoint", "cascade_fixpoint", "uneven_batch", "serving"],
+                    default=None,
+                    help="run only this measurement of the extras and print its JSON (for profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (read MIB + write MIB) and exit: the "
                          "reference kernel for scaling rocprof FETCH_SIZE / WRITE_SIZE counters")
@@ -68,17 +71,20 @@ def parse():
 PROFILES = {"sliced": os.path.join(REPO, "profiles", "k_sliced_wide_batched_sac.txt")}
 
 
+KERNEL_SOURCES = ("device.h", "kernels.h")  # every __global__ kernel and the device code it runs
+
+
 def kernel_source_fingerprint() -> str:
-    """sha256 over the engine's device and launch sources (cpim_amd/csrc), in
-    name order: a committed profile taken from other sources is stale."""
-    import glob
+    """sha256 over the engine's device sources (cpim_amd/csrc/device.h and
+    kernels.h, in that order): a committed profile taken from other device
+    code is stale.  Host-side changes (launch code, serving, bindings) leave
+    it unchanged."""
     import hashlib
     h = hashlib.sha256()
-    for path in sorted(glob.glob(os.path.join(REPO, "cpim_amd", "csrc", "*"))):
-        if path.endswith((".h", ".hip", ".inc")):
-            h.update(os.path.basename(path).encode())
-            with open(path, "rb") as f:
-                h.update(f.read())
+    for name in KERNEL_SOURCES:
+        h.update(name.encode())
+        with open(os.path.join(REPO, "cpim_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
     return h.hexdigest()
 
 
@@ -168,6 +174,15 @@ def single_fixpoint_one(desc, label: str, log) -> dict:
     return out
 
 
+def cascade_fixpoint(log) -> dict:
+    """The cascade instance of the single-fixpoint shape (t=0.892, 25 rounds,
+    13% of the values deleted)."""
+    from cpim_amd import workload
+    return single_fixpoint_one(
+        workload.single_large_cascade(), "root GAC fixpoint, random binary CSP n=2000 d=64 "
+        "e=40000 tightness 0.892, 10% of values pre-removed (multi-round cascade)", log)
+
+
 def single_fixpoint(log) -> dict:
     """Root GAC fixpoints of n=2000 d=64 e=40000 (10% of values pre-removed):
     the benchmark instance (t=0.25, one round, nothing deleted) and the
@@ -177,9 +192,7 @@ def single_fixpoint(log) -> dict:
         "benchmark": single_fixpoint_one(
             workload.single_large(), "root GAC fixpoint, random binary CSP n=2000 d=64 e=40000 "
             "tightness 0.25, 10% of values pre-removed", log),
-        "cascade": single_fixpoint_one(
-            workload.single_large_cascade(), "root GAC fixpoint, random binary CSP n=2000 d=64 "
-            "e=40000 tightness 0.892, 10% of values pre-removed (multi-round cascade)", log),
+        "cascade": cascade_fixpoint(log),
     }
 
 
@@ -230,6 +243,8 @@ def uneven_batch(count: int, log, per_instance_sample: int = 256) -> dict:
            "one_launch_ms": 1000.0 * one, "kernel_ms": kms,
            "fixpoints_per_s": (count + nodes) / one,
            "tuned_choice": choice, "tuning_s": tune_s,
+           "tuning_timings_s": [(t["choice"], t.get("seconds")) for t in
+                                autotune.load_cache().get(autotune.batch_key(models), {}).get("timings", [])],
            "per_instance_sample": len(sub),
            "per_instance_fixpoints_per_s": sub_fix / per,
            "speedup_vs_per_instance": ((count + nodes) / one) / (sub_fix / per)}
@@ -238,15 +253,20 @@ def uneven_batch(count: int, log, per_instance_sample: int = 256) -> dict:
     return out
 
 
-def serve_poisson(reqs, rate: float, node_limit: int, nodes_per_step: int, log, label: str) -> dict:
-    """Poisson arrivals of the prepared requests at `rate`/s into one server:
-    solves/s over the whole run, latency p50 / p99 from arrival to result."""
+def serve_poisson(reqs, count: int, rate: float, node_limit: int, nodes_per_step: int, log,
+                  label: str) -> dict:
+    """Poisson arrivals of `count` requests at `rate`/s into one server,
+    cycling through the prepared distinct requests (each submission is a
+    fresh request: its own upload, arena and search).  Reports solves/s over
+    the whole run, the sustained completion rate over the last 80% of the
+    arrival window (steady state: it matches the offered rate when the
+    server keeps up), the largest backlog, and latency p50 / p99 from arrival
+    to result."""
     from cpim_amd import Server
-    count = len(reqs)
     rng = np.random.default_rng(0x5EED2026)
     arrive = np.cumsum(rng.exponential(1.0 / rate, count))
     srv = Server(max_slots=256, nodes_per_step=nodes_per_step)
-    lat, limited, solved, unsat = [], 0, 0, 0
+    lat, done_t, limited, solved, unsat, backlog = [], [], 0, 0, 0, 0
     try:
         pending = {}
         nxt = 0
@@ -254,8 +274,9 @@ def serve_poisson(reqs, rate: float, node_limit: int, nodes_per_step: int, log, 
         while nxt < count or pending:
             now = time.perf_counter() - t0
             while nxt < count and arrive[nxt] <= now:
-                pending[srv.submit(reqs[nxt], node_limit=node_limit)] = arrive[nxt]
+                pending[srv.submit(reqs[nxt % len(reqs)], node_limit=node_limit)] = arrive[nxt]
                 nxt += 1
+            backlog = max(backlog, len(pending))
             if not pending:
                 time.sleep(max(0.0, arrive[nxt] - now))
                 continue
@@ -265,6 +286,7 @@ def serve_poisson(reqs, rate: float, node_limit: int, nodes_per_step: int, log, 
                 r = srv.result(rid)
                 srv.release(rid)
                 lat.append(done_at - pending.pop(rid))
+                done_t.append(done_at)
                 limited += r["status"] == 0
                 solved += r["status"] == 1
                 unsat += r["status"] == 2
@@ -273,12 +295,23 @@ def serve_poisson(reqs, rate: float, node_limit: int, nodes_per_step: int, log, 
     finally:
         srv.close()
     lat_ms = 1000.0 * np.array(lat)
-    out = {"solves_per_s": count / wall, "offered_per_s": rate, "p50_ms": float(np.percentile(lat_ms, 50)),
-           "p99_ms": float(np.percentile(lat_ms, 99)), "solved": int(solved), "unsat": int(unsat),
-           "node_limited": int(limited), "peak_device_bytes": int(peak)}
-    log(f"serving [{label}]: {out['solves_per_s']:.1f} solves/s offered {rate:g}/s, p50 {out['p50_ms']:.1f} ms, "
-        f"p99 {out['p99_ms']:.1f} ms")
+    t_end = float(arrive[-1])
+    t_beg = 0.2 * t_end
+    done_t = np.array(done_t)
+    sustained = float(((done_t >= t_beg) & (done_t <= t_end)).sum()) / max(1e-9, t_end - t_beg)
+    out = {"solves_per_s": count / wall, "offered_per_s": rate, "sustained_per_s": sustained,
+           "keeps_up": bool(sustained >= 0.9 * rate), "max_backlog": int(backlog),
+           "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+           "requests": int(count), "distinct_requests": len(reqs), "arrival_window_s": t_end,
+           "solved": int(solved), "unsat": int(unsat), "node_limited": int(limited),
+           "peak_device_bytes": int(peak)}
+    log(f"serving [{label}]: {count} requests, offered {rate:g}/s, sustained {sustained:.1f}/s "
+        f"({'keeps up' if out['keeps_up'] else 'falls behind'}, backlog <= {backlog}), "
+        f"{out['solves_per_s']:.1f} solves/s overall, p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms")
     return out
+
+
+C3_DISTINCT = 16  # distinct C3-shaped requests generated (164 MB of relations each)
 
 
 def serving(count: int, rate: float, c3_count: int, c3_rate: float, log) -> dict:
@@ -289,19 +322,20 @@ def serving(count: int, rate: float, c3_count: int, c3_rate: float, log) -> dict
     from cpim_amd import workload
     out = {}
     reqs = [workload.serving_request(i) for i in range(count)]
-    small = serve_poisson(reqs, rate, 5000, 64, log, "small")
+    small = serve_poisson(reqs, count, rate, 5000, 64, log, "small")
     small["workload"] = (f"{count} requests, Poisson arrivals at {rate:g}/s; each a MAC solve (node limit "
                          "5000) of n~U[20,40] d=16 binary CSP + 4 arity-3 tables of 500 tuples")
     out["small"] = small
     if c3_count > 0:
+        distinct = min(c3_count, C3_DISTINCT)
         t = time.perf_counter()
         with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-            reqs = list(ex.map(workload.serving_request_c3, range(c3_count)))
-        log(f"serving [c3]: {c3_count} requests generated in {time.perf_counter() - t:.1f} s")
-        c3 = serve_poisson(reqs, c3_rate, 64, 16, log, "c3")
-        c3["workload"] = (f"{c3_count} requests, Poisson arrivals at {c3_rate:g}/s; each a MAC search with a "
-                          "budget of 64 nodes on a random binary CSP n=500 d=256 e=20000 tightness 0.5 + 4 "
-                          "arity-3 tables of 500 tuples over 16 values")
+            reqs = list(ex.map(workload.serving_request_c3, range(distinct)))
+        log(f"serving [c3]: {distinct} distinct requests generated in {time.perf_counter() - t:.1f} s")
+        c3 = serve_poisson(reqs, c3_count, c3_rate, 64, 16, log, "c3")
+        c3["workload"] = (f"{c3_count} requests ({distinct} distinct), Poisson arrivals at {c3_rate:g}/s; each "
+                          "a MAC search with a budget of 64 nodes on a random binary CSP n=500 d=256 e=20000 "
+                          "tightness 0.5 + 4 arity-3 tables of 500 tuples over 16 values")
         out["c3_shaped"] = c3
     return out
 
@@ -319,6 +353,14 @@ def main():
     from cpim_amd.model import domains_to_sets
 
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
+    if args.only:
+        fn = {"single_fixpoint": lambda: single_fixpoint(log),
+              "cascade_fixpoint": lambda: cascade_fixpoint(log),
+              "uneven_batch": lambda: uneven_batch(args.batch_instances, log),
+              "serving": lambda: serving(args.serve_requests, args.serve_rate,
+                                         args.serve_c3_requests, args.serve_c3_rate, log)}[args.only]
+        print(json.dumps({args.only: fn()}), flush=True)
+        return
     t0 = time.time()
     desc = workload.batched_sac()
     n, W = desc.num_vars, desc.words

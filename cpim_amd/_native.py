@@ -213,6 +213,7 @@ GPU_SIGNATURES = {
     "cpim_gmodel_write_domains": (C.c_int, [_VP, C.c_int32, p_u64]),
     "cpim_gmodel_enforce_sac": (C.c_int, [_VP, C.c_int32, C.POINTER(CSacStats)]),
     "cpim_gmodel_singleton_tests": (C.c_int, [_VP, p_i32, C.c_int64, C.c_int32, p_u8, C.POINTER(CSacStats)]),
+    "cpim_gmodel_weights": (C.c_int, [_VP, C.POINTER(C.c_float), C.c_int64, p_i64, C.c_int32]),
     "cpim_gmodel_solve": (C.c_int, [_VP, C.POINTER(CSolveOpts), C.POINTER(CSearchStats), p_i32,
                                     C.POINTER(CNodeRecord), p_i32]),
     "cpim_batch_create": (C.c_int, [C.POINTER(CModelDesc), C.c_int32, C.POINTER(CGModelOptions), C.POINTER(_VP)]),

@@ -31,15 +31,16 @@ from . import _native as N
 Choice = Dict[str, int]
 
 # Candidates.  SAC: kernel 0 is the shared-stage kernel (masks 1, 2 or 4 per
-# value, nibble tables on or off), kernel 1 the one-stage-per-wave kernel.
+# value; nibble tables on or off; with them, none, 4 or 6 byte tables in
+# front), kernel 1 the one-stage-per-wave kernel.
 SAC_SPACE: List[Choice] = (
-    [{"sac_slice_kernel": 0, "sac_slice_masks": m, "sac_slice_nibbles": nb}
-     for m in (1, 2, 4) for nb in (1, 2)] +
-    [{"sac_slice_kernel": 1, "sac_slice_masks": 0, "sac_slice_nibbles": 0}])
+    [{"sac_slice_kernel": 0, "sac_slice_masks": m, "sac_slice_nibbles": nb, "sac_slice_bytes": by}
+     for m in (1, 2, 4) for nb, by in ((1, 1), (1, 4), (1, 6), (2, 1))] +
+    [{"sac_slice_kernel": 1, "sac_slice_masks": 0, "sac_slice_nibbles": 0, "sac_slice_bytes": 0}])
 # MAC unit batches: waves per unit and the network image in LDS.
 BATCH_SPACE: List[Choice] = [{"unit_threads": t, "mac_lds_net": lds}
                              for t in (256, 512, 1024) for lds in (0, 1)]
-SAC_KNOBS = ("sac_slice_kernel", "sac_slice_masks", "sac_slice_nibbles")
+SAC_KNOBS = ("sac_slice_kernel", "sac_slice_masks", "sac_slice_nibbles", "sac_slice_bytes")
 BATCH_KNOBS = ("unit_threads", "mac_lds_net")
 
 
@@ -165,7 +166,8 @@ def tuned_sac(model, tests: np.ndarray, path: Optional[str] = None) -> Choice:
     return entry["choice"]
 
 
-def _batch_key(models) -> str:
+def batch_key(models) -> str:
+    """The tuning bucket of a batch of models."""
     words = max(m.words for m in models)
     return batch_bucket(words, max(m.max_dom for m in models), max(m.num_cons for m in models), len(models))
 
@@ -175,7 +177,7 @@ def tune_batch(batch, node_limit: int, reps: int = 1, space: Optional[List[Choic
     """Time every candidate on batch.run(node_limit) and keep the fastest for
     the batch's bucket.  Every candidate must give the same outcome per unit
     (status, node counts and, with trace_capacity, the node sequence)."""
-    bucket = _batch_key(batch.models)
+    bucket = batch_key(batch.models)
 
     def run():
         res, _ = batch.run(node_limit, trace_capacity=trace_capacity)
@@ -188,7 +190,7 @@ def tune_batch(batch, node_limit: int, reps: int = 1, space: Optional[List[Choic
 
 
 def tuned_batch(batch, node_limit: int, path: Optional[str] = None) -> Choice:
-    bucket = _batch_key(batch.models)
+    bucket = batch_key(batch.models)
     entry = load_cache(path).get(bucket) or tune_batch(batch, node_limit, path=path)
     apply(entry["choice"])
     return entry["choice"]

@@ -603,6 +603,28 @@ struct cpim_gmodel {
   DevProps props;
   SacScratch sac;
   hipEvent_t gev[2] = {nullptr, nullptr};  // kernel timing of the fixpoint entry points
+  // dom/wdeg bookkeeping of the network (Tabular::weight, Network.cpp:303):
+  // weights start at 1 with the model and persist across solves, as the
+  // reference's constraint weights do; the stamps and clock order AC3's
+  // revisions across calls
+  DevAlloc wdeg_weight, wdeg_stamp, wdeg_clock;
+
+  // the weights, stamps and clock, allocated and set to their first values on
+  // first use or on reset
+  void wdeg_init() {
+    const int nc = std::max(1, dm.num_active);
+    if (!wdeg_weight.p) {
+      wdeg_weight = DevAlloc(sizeof(float) * nc);
+      wdeg_stamp = DevAlloc(sizeof(long long) * nc);
+      wdeg_clock = DevAlloc(sizeof(long long));
+    }
+    const std::vector<float> ones(nc, 1.0f);
+    const long long one = 1;
+    HIPC(hipMemcpyAsync(wdeg_weight.p, ones.data(), sizeof(float) * nc, hipMemcpyHostToDevice, stream));
+    HIPC(hipMemsetAsync(wdeg_stamp.p, 0, sizeof(long long) * nc, stream));
+    HIPC(hipMemcpyAsync(wdeg_clock.p, &one, sizeof(one), hipMemcpyHostToDevice, stream));
+    HIPC(hipStreamSynchronize(stream));  // `ones` and `one` are host stack memory
+  }
 
   void gac_events() {
     for (hipEvent_t& e : gev)
@@ -1232,6 +1254,26 @@ int cpim_gmodel_read_domains(const cpim_gmodel* m, int32_t level, uint64_t* out)
   });
 }
 
+int cpim_gmodel_weights(cpim_gmodel* m, float* out, int64_t capacity, int64_t* count, int32_t reset) {
+  return guarded([&] {
+    REQUIRE(m && count, "null argument");
+    REQUIRE(capacity >= 0 && (capacity == 0 || out), "null argument");
+    DeviceGuard g(m->device);
+    const int nc = m->dm.num_active;
+    *count = nc;
+    if (out && capacity > 0 && nc > 0) {
+      const long long k = std::min<long long>(capacity, nc);
+      if (m->wdeg_weight.p) {
+        HIPC(hipMemcpyAsync(out, m->wdeg_weight.p, sizeof(float) * k, hipMemcpyDeviceToHost, m->stream));
+        HIPC(hipStreamSynchronize(m->stream));
+      } else {
+        for (long long i = 0; i < k; ++i) out[i] = 1.0f;  // no dom/wdeg solve yet
+      }
+    }
+    if (reset && m->wdeg_weight.p) m->wdeg_init();
+  });
+}
+
 int cpim_gmodel_write_domains(cpim_gmodel* m, int32_t level, const uint64_t* in) {
   return guarded([&] {
     REQUIRE(m && in, "null argument");
@@ -1839,21 +1881,11 @@ int cpim_gmodel_solve(cpim_gmodel* m, const cpim_solve_opts* opts, cpim_search_s
     A.deadline = 0;
     A.root_out = root.as<u64>();
     A.nbr = o.var_heuristic == CPIM_VARH_DOM_DEG ? m->dm.nbr.as<int>() : nullptr;
-    DevAlloc wdeg_weight, wdeg_stamp, wdeg_clock;
     if (o.var_heuristic == CPIM_VARH_DOM_WDEG) {
-      const int nc = std::max(1, m->dm.num_active);
-      const std::vector<float> ones(nc, 1.0f);
-      const long long one = 1;
-      wdeg_weight = DevAlloc(sizeof(float) * nc);
-      wdeg_stamp = DevAlloc(sizeof(long long) * nc);
-      wdeg_clock = DevAlloc(sizeof(long long));
-      HIPC(hipMemcpyAsync(wdeg_weight.p, ones.data(), sizeof(float) * nc, hipMemcpyHostToDevice, m->stream));
-      HIPC(hipMemsetAsync(wdeg_stamp.p, 0, sizeof(long long) * nc, m->stream));
-      HIPC(hipMemcpyAsync(wdeg_clock.p, &one, sizeof(one), hipMemcpyHostToDevice, m->stream));
-      A.weight = wdeg_weight.as<float>();
-      A.stamp_tab = wdeg_stamp.as<long long>();
-      A.clock = wdeg_clock.as<long long>();
-      HIPC(hipStreamSynchronize(m->stream));  // `ones` and `one` are host stack memory
+      if (!m->wdeg_weight.p) m->wdeg_init();
+      A.weight = m->wdeg_weight.as<float>();
+      A.stamp_tab = m->wdeg_stamp.as<long long>();
+      A.clock = m->wdeg_clock.as<long long>();
     }
     DevNet net = m->dm.net();
     const long long bytes = wg_mem_bytes(n, m->dm.W);

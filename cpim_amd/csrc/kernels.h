@@ -2316,7 +2316,8 @@ struct UnitDesc {
   MacArgs args;
   long long scratch_off;  // per-unit global scratch when the domain does not fit in LDS
   int use_lds;            // 1: working memory in LDS; 2: network image in LDS as well
-  int pad;
+  int reset;              // 1: start the search over: level 0 = init, state = MAC_INIT
+  const u64* init;        // the unit's initial domains, [n][W]
   LdsNet lnet;
 };
 
@@ -2334,6 +2335,18 @@ __global__ void k_mac_units(const UnitDesc* __restrict__ units, int count, unsig
     __syncthreads();
     if (u >= count) break;
     const UnitDesc d = units[u];
+    if (uniform_int(d.reset)) {
+      // a new run of the batch: the workgroup restores level 0 and the state
+      // itself, which spares the host two copies per unit
+      const long long nw = (long long)d.net.n * d.net.W;
+      for (long long i = threadIdx.x; i < nw; i += blockDim.x) d.args.trail[i] = d.init[i];
+      if (threadIdx.x == 0) {
+        MacState s0{};
+        s0.status = MAC_INIT;
+        *d.args.state = s0;
+      }
+      __syncthreads();
+    }
     // One inlined copy of the search per place of the unit's memory (network
     // image and working memory in LDS, working memory in LDS, global
     // scratch), so that LDS accesses compile to ds_read / ds_write: one copy

@@ -246,6 +246,20 @@ class GModel:
         N.check(N.gpu_lib().cpim_gmodel_next_value(self.handle, var, value, level, C.byref(v)))
         return v.value
 
+    def Weights(self, reset: bool = False) -> np.ndarray:
+        """dom/wdeg constraint weights (one per propagated constraint, in
+        constraint order).  They start at 1 with the model and persist across
+        solves, as Tabular::weight does in the reference's network
+        (Network.cpp:303, AC3.cpp:122-126); reset=True sets them back to 1
+        after reading them."""
+        lib = N.gpu_lib()
+        k = C.c_int64(0)
+        N.check(lib.cpim_gmodel_weights(self._h, None, 0, C.byref(k), 0))
+        w = np.zeros(max(1, k.value), np.float32)
+        N.check(lib.cpim_gmodel_weights(self._h, w.ctypes.data_as(C.POINTER(C.c_float)), k.value,
+                                        C.byref(k), 1 if reset else 0))
+        return w[:k.value]
+
     def ReadDomains(self, level: Optional[int] = None) -> np.ndarray:
         """Host copy of one level ([num_vars, dom_words] uint64)."""
         if level is None:

@@ -154,6 +154,16 @@ class GModel {
     check(cpim_gmodel_read_domains(h_, level, out.data()));
     return out;
   }
+  // dom/wdeg constraint weights, one per propagated constraint; they start at
+  // 1 with the model and persist across solves (Tabular::weight); reset sets
+  // them back to 1 after the copy.
+  std::vector<float> Weights(bool reset = false) {
+    int64_t k = 0;
+    check(cpim_gmodel_weights(h_, nullptr, 0, &k, 0));
+    std::vector<float> w((size_t)k);
+    check(cpim_gmodel_weights(h_, w.data(), k, &k, reset ? 1 : 0));
+    return w;
+  }
   cpim_sac_stats EnforceSAC(bool batched = true) {
     cpim_sac_stats st{};
     check(cpim_gmodel_enforce_sac(h_, batched ? CPIM_SAC_BATCHED : CPIM_SAC_PLAIN, &st));

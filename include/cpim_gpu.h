@@ -271,8 +271,10 @@ int cpim_gmodel_singleton_tests(cpim_gmodel* model, const int32_t* tests, int64_
  *   DOM_WDEG  key = |D(x)| / x_w in double, x_w = the summed weights of the
  *             constraints on x with another undecided variable; -1 when
  *             |D(x)| = 1 or x_w = 0 (Heuristic::VRH_DOM_WDEG_MIN).  Weights
- *             start at 1 on every solve and grow by one on the constraint
- *             that empties a domain first in AC3's order (src/AC3.cpp:122-126);
+ *             start at 1 with the model, persist across solves (read and
+ *             reset them with cpim_gmodel_weights) and grow by one on the
+ *             constraint that empties a domain first in AC3's order
+ *             (src/AC3.cpp:122-126);
  *             with this ordering every fixpoint of the search runs in that
  *             order, on one wave, so it is slower than the other two.
  * Neighbours are the distinct other variables of the constraints on x
@@ -281,6 +283,17 @@ int cpim_gmodel_singleton_tests(cpim_gmodel* model, const int32_t* tests, int64_
 #define CPIM_VARH_DOM_MIN 0
 #define CPIM_VARH_DOM_DEG 1
 #define CPIM_VARH_DOM_WDEG 2
+
+/* dom/wdeg constraint weights (Tabular::weight, include/Network.h:243): one
+ * float per propagated constraint, in constraint order; they start at 1 with
+ * the model (Network.cpp:303) and persist across cpim_gmodel_solve calls,
+ * as the reference's weights persist in its network; a solve under
+ * CPIM_VARH_DOM_WDEG raises them (src/AC3.cpp:122-126).  Copies up to
+ * `capacity` weights into out (out may be NULL when capacity is 0) and sets
+ * *count to the number of propagated constraints; reset != 0 then sets every
+ * weight back to 1.  No counterpart in the reference's GPU interface (its
+ * weights are a public member of each constraint). */
+int cpim_gmodel_weights(cpim_gmodel* model, float* out, int64_t capacity, int64_t* count, int32_t reset);
 
 typedef struct cpim_solve_opts {
   int64_t time_limit_ms; /* 0 = unlimited */

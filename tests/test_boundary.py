@@ -155,7 +155,7 @@ def test_serving_request_shape():
 def test_tuning_knobs():
     from cpim_amd import _native as N
     for name in ("sac_slice_masks", "sac_slice_threads", "sac_slice_kernel", "sac_slice_nibbles",
-                 "unit_threads", "mac_lds_net"):
+                 "sac_slice_bytes", "unit_threads", "mac_lds_net"):
         old = N.tune_get(name)
         N.tune_set(name, 0)
         assert N.tune_get(name) == 0

@@ -32,8 +32,9 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 GAC_MODES = {"plain": N.CPIM_GAC_PLAIN, "persistent": N.CPIM_GAC_PERSISTENT,
              "async": N.CPIM_GAC_ASYNC, "block": N.CPIM_GAC_BLOCK}
-SLICE_VARIANTS = [(0, 0, 0), (0, 1, 1), (0, 2, 1), (0, 4, 1), (0, 1, 2), (0, 2, 2), (0, 4, 2), (1, 0, 0)]
-SLICE_KNOBS = ("sac_slice_kernel", "sac_slice_masks", "sac_slice_nibbles")
+SLICE_VARIANTS = [(0, 0, 0, 0), (0, 1, 1, 0), (0, 2, 1, 0), (0, 4, 1, 0), (0, 1, 2, 0), (0, 2, 2, 0),
+                  (0, 4, 2, 0), (0, 1, 1, 6), (0, 2, 1, 4), (0, 4, 1, 4), (0, 4, 1, 6), (1, 0, 0, 0)]
+SLICE_KNOBS = ("sac_slice_kernel", "sac_slice_masks", "sac_slice_nibbles", "sac_slice_bytes")
 
 
 def popcount(dom):

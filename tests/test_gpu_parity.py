@@ -239,10 +239,12 @@ SLICED = [  # (n, d, e, p, preremove, tests)
 ]
 
 
-# (kernel, masks, nibbles) knobs: default, each mask width with and without
-# nibble tables, and the one-stage-per-wave kernel
-SLICE_VARIANTS = [(0, 0, 0), (0, 1, 1), (0, 2, 1), (0, 4, 1), (0, 1, 2), (0, 2, 2), (0, 4, 2), (1, 0, 0)]
-SLICE_KNOBS = ("sac_slice_kernel", "sac_slice_masks", "sac_slice_nibbles")
+# (kernel, masks, nibbles, bytes) knobs: default, each mask width with and
+# without nibble tables, byte tables in front of them, and the
+# one-stage-per-wave kernel
+SLICE_VARIANTS = [(0, 0, 0, 0), (0, 1, 1, 0), (0, 2, 1, 0), (0, 4, 1, 0), (0, 1, 2, 0), (0, 2, 2, 0),
+                  (0, 4, 2, 0), (0, 1, 1, 6), (0, 2, 1, 4), (0, 4, 1, 4), (0, 4, 1, 6), (1, 0, 0, 0)]
+SLICE_KNOBS = ("sac_slice_kernel", "sac_slice_masks", "sac_slice_nibbles", "sac_slice_bytes")
 
 
 def set_slice_knobs(values):
@@ -250,11 +252,11 @@ def set_slice_knobs(values):
         N.tune_set(k, v)
 
 
-@pytest.fixture(params=SLICE_VARIANTS, ids=lambda v: f"kernel{v[0]}-masks{v[1]}-nib{v[2]}")
+@pytest.fixture(params=SLICE_VARIANTS, ids=lambda v: f"kernel{v[0]}-masks{v[1]}-nib{v[2]}-bytes{v[3]}")
 def slice_variant(request):
     set_slice_knobs(request.param)
     yield request.param
-    set_slice_knobs((0, 0, 0))
+    set_slice_knobs((0, 0, 0, 0))
 
 
 @pytest.mark.parametrize("name", golden_names())
@@ -568,10 +570,49 @@ def test_batched_sac_full_size_sample(gpu):
             try:
                 got, sst = m.SingletonTests(sample, mode="sliced")
             finally:
-                set_slice_knobs((0, 0, 0))
+                set_slice_knobs((0, 0, 0, 0))
             assert (got == want).all(), variant
         # every singleton test of the full pass, in one launch
         allf, sst = m.SingletonTests(tests)
         assert sst.singleton_tests == len(tests)
         assert (allf[np.searchsorted(tests[:, 0] * 256 + tests[:, 1],
                                      sample[:, 0] * 256 + sample[:, 1])] == want).all()
+
+
+# (n, d, e, p, seed): networks whose first dom/wdeg search finds a solution
+# after some failures, and whose second search, started from the raised
+# weights, takes another path
+WDEG_AGAIN = [(30, 16, 120, 0.40, 1300), (30, 16, 120, 0.42, 5021), (30, 16, 120, 0.42, 5038)]
+
+
+@pytest.mark.parametrize("case", range(len(WDEG_AGAIN)))
+def test_dom_wdeg_weights_persist_across_solves(gpu, case):
+    # the weights live in the model as Tabular::weight lives in the network
+    # (Network.cpp:303, AC3.cpp:122-126): a second solve continues from the
+    # weights the first one raised, as the oracle's second search on the same
+    # network does; after a reset the first search repeats.  (Both searches
+    # end at a solution: after an exhausted or node-limited search the CPU
+    # network and the device model are left in different states, so a
+    # second search there would not compare weights alone.)
+    n, d, e, p, seed = WDEG_AGAIN[case]
+    desc = workload.random_binary(n, d, e, p, seed)
+    with oracle.var_heuristic(2):
+        o = oracle.Oracle(desc)
+        r1 = o.mac(node_limit=5000, trace_cap=10000)
+        w1 = o.weights().copy()
+        r2 = o.mac(node_limit=5000, trace_cap=10000)
+        w2 = o.weights().copy()
+    assert r1.num_sol and r2.num_sol
+    assert (w1 > 1).any() and (w2 != w1).any() and r1.trace != r2.trace
+    with GModelAdapter.Build(desc) as m:
+        assert (m.Weights() == 1).all() and len(m.Weights()) == len(w1)
+        st, sol, trace = m.Solve(node_limit=5000, trace_capacity=10000, heuristic="dom/wdeg")
+        assert trace == r1.trace and (m.Weights() == w1).all()
+        st, sol, trace = m.Solve(node_limit=5000, trace_capacity=10000, heuristic="dom/wdeg")
+        assert (st.num_positive, st.num_negative) == (r2.num_positive, r2.num_negative)
+        assert trace == r2.trace and (m.Weights() == w2).all()
+        assert list(sol) == list(r2.solution)
+        assert (m.Weights(reset=True) == w2).all()
+        assert (m.Weights() == 1).all()
+        st, sol, trace = m.Solve(node_limit=5000, trace_capacity=10000, heuristic="dom/wdeg")
+        assert trace == r1.trace and (m.Weights() == w1).all()
