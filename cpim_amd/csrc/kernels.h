@@ -1528,6 +1528,24 @@ __device__ __forceinline__ const u64* nib_base(const u64* stage) {
   return stage + (256ll * NB - 32ll * NB) * M;
 }
 
+// The M masks of one (variable, value) with plain vector loads (32 bytes for
+// M = 4), issued together.
+template <int M>
+__device__ __forceinline__ void load_masks(const u64* p, u64 (&out)[M]) {
+  if constexpr (M % 2 == 0) {
+    const ulonglong2* q = reinterpret_cast<const ulonglong2*>(p);
+#pragma unroll
+    for (int h = 0; h < M / 2; ++h) {
+      const ulonglong2 v = q[h];
+      out[2 * h] = v.x;
+      out[2 * h + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < M; ++m) out[m] = p[m];
+  }
+}
+
 template <int M, bool kNib, int NB>
 __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64* slu, const u64* stage,
                                             const u64* atw, int W, int dpad, const u64 (&act)[M],
@@ -1541,9 +1559,15 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
     const u64 flip = in ? ~0ull : 0ull;  // a set slice bit flips the base state
     u64 pend[M];
     if (touched_u) {
+      // plain loads, all M masks at once: another wave may be removing
+      // values of u at the same time, and a load that misses those removals
+      // only makes pend a superset of the live tests (the alive sets only
+      // shrink), which costs lookups but not exactness, since a removal takes
+      // the change from the atomic's returned word
+      u64 sv[M];
+      load_masks<M>(&slu[(long long)b * M], sv);
 #pragma unroll
-      for (int m = 0; m < M; ++m)
-        pend[m] = all[m] & act[m] & (ld_wg(&slu[(long long)b * M + m]) ^ flip);
+      for (int m = 0; m < M; ++m) pend[m] = all[m] & act[m] & (sv[m] ^ flip);
     } else {
 #pragma unroll
       for (int m = 0; m < M; ++m) pend[m] = in ? (all[m] & act[m]) : 0ull;
