@@ -51,6 +51,8 @@ def parse():
                     help="skip the single-fixpoint, uneven-batch and serving measurements")
     ap.add_argument("--batch-instances", type=int, default=4096,
                     help="instances of the 4096-instance uneven batch timed in the extras")
+    ap.add_argument("--per-instance-sample", type=int, default=256,
+                    help="uneven instances also timed one launch each (0 = skip that comparison)")
     ap.add_argument("--serve-requests", type=int, default=2048)
     ap.add_argument("--serve-rate", type=float, default=1000.0,
                     help="Poisson arrival rate of serving requests (requests/s)")
@@ -218,38 +220,41 @@ def uneven_batch(count: int, log, per_instance_sample: int = 256) -> dict:
         t = time.perf_counter()
         res, kms = b.run(64, persistent=True)
         one = time.perf_counter() - t
+        heavy = b.heavy_units()
     finally:
         b.close()
     nodes = sum(r.num_positive + r.num_negative for r in res)
-    # one launch per instance, on a spread sample
-    idx = np.linspace(0, count - 1, min(per_instance_sample, count)).astype(np.int64)
-    sub = [models[i] for i in idx]
-    bs = Batch(sub)
-    try:
-        bs.run(64, persistent=False)
-        t = time.perf_counter()
-        rs, _ = bs.run(64, persistent=False)
-        per = time.perf_counter() - t
-    finally:
-        bs.close()
-    for i, r in zip(idx, rs):
-        a = res[i]
-        assert (a.status, a.num_positive, a.num_negative) == (r.status, r.num_positive, r.num_negative), \
-            "one launch and one launch per instance disagree"
-    sub_fix = len(sub) + sum(r.num_positive + r.num_negative for r in rs)
     out = {"workload": f"{count} uneven instances (n~U[20,200], d in {{8..128}}, density~U[0.1,0.5], "
                        "phase-transition tightness): root fixpoint + first 64 MAC nodes each",
            "instances": count, "nodes": int(nodes),
            "one_launch_ms": 1000.0 * one, "kernel_ms": kms,
            "fixpoints_per_s": (count + nodes) / one,
+           "heavy_units": heavy,
            "tuned_choice": choice, "tuning_s": tune_s,
            "tuning_timings_s": [(t["choice"], t.get("seconds")) for t in
-                                autotune.load_cache().get(autotune.batch_key(models), {}).get("timings", [])],
-           "per_instance_sample": len(sub),
-           "per_instance_fixpoints_per_s": sub_fix / per,
-           "speedup_vs_per_instance": ((count + nodes) / one) / (sub_fix / per)}
-    log(f"uneven batch: one launch {out['one_launch_ms']:.1f} ms ({out['fixpoints_per_s']:,.0f} fixpoints/s), "
-        f"per instance {out['per_instance_fixpoints_per_s']:,.0f} fixpoints/s, tuned {choice}")
+                                autotune.load_cache().get(autotune.batch_key(models), {}).get("timings", [])]}
+    if per_instance_sample > 0:
+        # one launch per instance, on a spread sample
+        idx = np.linspace(0, count - 1, min(per_instance_sample, count)).astype(np.int64)
+        sub = [models[i] for i in idx]
+        bs = Batch(sub)
+        try:
+            bs.run(64, persistent=False)
+            t = time.perf_counter()
+            rs, _ = bs.run(64, persistent=False)
+            per = time.perf_counter() - t
+        finally:
+            bs.close()
+        for i, r in zip(idx, rs):
+            a = res[i]
+            assert (a.status, a.num_positive, a.num_negative) == (r.status, r.num_positive, r.num_negative), \
+                "one launch and one launch per instance disagree"
+        sub_fix = len(sub) + sum(r.num_positive + r.num_negative for r in rs)
+        out.update({"per_instance_sample": len(sub), "per_instance_fixpoints_per_s": sub_fix / per,
+                    "speedup_vs_per_instance": ((count + nodes) / one) / (sub_fix / per)})
+    log(f"uneven batch: one launch {out['one_launch_ms']:.1f} ms ({out['fixpoints_per_s']:,.0f} fixpoints/s, "
+        f"{heavy} heavy units apart), per instance "
+        f"{out.get('per_instance_fixpoints_per_s', float('nan')):,.0f} fixpoints/s, tuned {choice}")
     return out
 
 
@@ -356,7 +361,7 @@ def main():
     if args.only:
         fn = {"single_fixpoint": lambda: single_fixpoint(log),
               "cascade_fixpoint": lambda: cascade_fixpoint(log),
-              "uneven_batch": lambda: uneven_batch(args.batch_instances, log),
+              "uneven_batch": lambda: uneven_batch(args.batch_instances, log, args.per_instance_sample),
               "serving": lambda: serving(args.serve_requests, args.serve_rate,
                                          args.serve_c3_requests, args.serve_c3_rate, log)}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
@@ -475,7 +480,8 @@ def main():
     }
     if not args.skip_extras:
         for key, fn in (("single_fixpoint", lambda: single_fixpoint(log)),
-                        ("uneven_batch", lambda: uneven_batch(args.batch_instances, log)),
+                        ("uneven_batch", lambda: uneven_batch(args.batch_instances, log,
+                                                              args.per_instance_sample)),
                         ("serving", lambda: serving(args.serve_requests, args.serve_rate,
                                                     args.serve_c3_requests, args.serve_c3_rate, log))):
             try:

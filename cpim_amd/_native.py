@@ -221,6 +221,7 @@ GPU_SIGNATURES = {
     "cpim_batch_run": (C.c_int, [_VP, C.c_int64, C.c_int32, C.POINTER(CBatchResult), C.POINTER(CNodeRecord),
                                  C.c_int32, C.POINTER(C.c_double)]),
     "cpim_batch_read_root": (C.c_int, [_VP, C.c_int32, p_u64]),
+    "cpim_batch_heavy_units": (C.c_int, [_VP, p_i32]),
     "cpim_server_create": (C.c_int, [C.POINTER(CServerOpts), C.POINTER(_VP)]),
     "cpim_server_destroy": (C.c_int, [_VP]),
     "cpim_server_submit": (C.c_int, [_VP, C.POINTER(CModelDesc), C.c_int64, p_i64]),

@@ -577,6 +577,13 @@ class Batch:
                                    r.num_negative, trace))
         return out, ms.value
 
+    def heavy_units(self) -> int:
+        """Units of the last run that ran in the separate 1024-thread launch
+        for the batch's heaviest searches."""
+        k = C.c_int32(0)
+        N.check(N.gpu_lib().cpim_batch_heavy_units(self._h, C.byref(k)))
+        return k.value
+
     def root_domains(self, index: int) -> np.ndarray:
         m = self.models[index]
         out = np.zeros((m.num_vars, m.words), np.uint64)

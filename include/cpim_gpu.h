@@ -372,6 +372,13 @@ int cpim_batch_run(cpim_batch* batch, int64_t node_limit, int32_t mode,
 /* Root-fixpoint domains of instance index ([num_vars][dom_words]). */
 int cpim_batch_read_root(const cpim_batch* batch, int32_t index, uint64_t* out);
 
+/* Units of the last cpim_batch_run (persistent mode, first launch) that ran
+ * in a launch of their own with 1024 threads per unit, beside the others:
+ * units whose estimated work (arcs x words x largest domain) exceeds 1.5
+ * times the mean work per resident workgroup, in batches of more units than
+ * CUs.  No counterpart in the reference. */
+int cpim_batch_heavy_units(const cpim_batch* batch, int32_t* count);
+
 /* ------------------------------------------------------------------------ */
 /* Serving: continuous batching of MAC solve requests                        */
 /* ------------------------------------------------------------------------ */

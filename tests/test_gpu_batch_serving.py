@@ -68,6 +68,36 @@ def test_uneven_batch_rerun_is_identical(gpu, uneven):
         b.close()
 
 
+def test_uneven_batch_heavy_units_launch_apart(gpu):
+    # more units than CUs, four of them (the largest uneven instances, n ~ 200,
+    # d = 128) far heavier than the mean work per workgroup slot: those run in
+    # a 1024-thread launch of their own beside the others; every unit still
+    # equals the oracle and the one-launch-per-unit path
+    from cpim_amd import _native as N
+    heavy_idx = [39, 3722, 572, 1817]
+    small = [workload.random_binary(12, 4, 20, 0.3, 4000 + i) for i in range(300)]
+    models = [workload.uneven_instance(i) for i in heavy_idx] + small
+    before = N.tune_get("unit_threads")
+    N.tune_set("unit_threads", 512)
+    b = Batch(models)
+    try:
+        res, _ = b.run(NODES, trace_capacity=2 * NODES)
+        assert b.heavy_units() == len(heavy_idx)
+        for i in list(range(len(heavy_idx))) + list(range(len(heavy_idx), len(models), 15)):
+            r = oracle.Oracle(models[i]).mac(node_limit=NODES, trace_cap=2 * NODES)
+            got = res[i]
+            assert (got.num_positive, got.num_negative) == (r.num_positive, r.num_negative), i
+            assert got.status == oracle_status(r, NODES), i
+            assert got.trace == r.trace, i
+        per, _ = b.run(NODES, persistent=False, trace_capacity=2 * NODES)
+        assert b.heavy_units() == 0
+        assert [(x.status, x.num_positive, x.num_negative, x.trace) for x in res] == \
+               [(x.status, x.num_positive, x.num_negative, x.trace) for x in per]
+    finally:
+        b.close()
+        N.tune_set("unit_threads", before)
+
+
 def solve_alone(desc, node_limit):
     with GModelAdapter.Build(desc, GModelOptions(skip_non_binary=False)) as m:
         st, sol, _ = m.Solve(node_limit=node_limit)
