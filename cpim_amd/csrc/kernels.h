@@ -1553,8 +1553,13 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
                                             const u64 (&all)[M]) {
   const int lane = lane_id();
   bool changed = false;
+  // word 0 of the next chunk's rows is loaded one chunk ahead, so that its
+  // latency overlaps this chunk's lookups
+  u64 row_next = plane_u[lane];
   for (int j = 0; j < W; ++j) {
     const int b = j * 64 + lane;
+    const u64 row0 = row_next;
+    if (j + 1 < W) row_next = plane_u[b + 64];
     const bool in = (base_u[j] >> lane) & 1ull;
     const u64 flip = in ? ~0ull : 0ull;  // a set slice bit flips the base state
     u64 pend[M];
@@ -1572,7 +1577,6 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
 #pragma unroll
       for (int m = 0; m < M; ++m) pend[m] = in ? (all[m] & act[m]) : 0ull;
     }
-    const u64 row0 = plane_u[b];
     const u64* nib = nib_base<M, NB>(stage);
     if (any_mask<M>(pend)) {
       const u64 r = row0 & atw[0];
