@@ -108,6 +108,8 @@ constexpr long long kLdsBudget = 60 * 1024;
 // Device wall-clock budget of one asynchronous fixpoint launch (ms); a
 // launch that exceeds it returns an error instead of running on.
 constexpr long long kAsyncBudgetMs = 20000;
+// Waves per CU of the asynchronous fixpoint's grid.
+constexpr int kAsyncWavesPerCu = 2;
 
 // MAC nodes one launch may run before it returns to the host.
 constexpr long long kNodesPerLaunch = 1 << 14;
@@ -949,8 +951,11 @@ void gac_async(cpim_gmodel* m, int assigned_var, cpim_gac_stats* st) {
   hipLaunchKernelGGL(k_seed_async, dim3(assigned_var < 0 ? (nfw + 255) / 256 : 1), dim3(256), 0, m->stream, F, nfw,
                      m->dm.num_active, m->dm.adj_off.as<int>(), m->dm.adj.as<int4>(), assigned_var, ctl);
   HIPC(hipGetLastError());
+  // two waves per CU: on the C2 cascade 1024 waves took 5.0-6.0 ms, 512
+  // waves 3.8-4.4 ms and 256 waves 4.4-4.9 ms (more waves, more contention on
+  // the frontier words and the pending count)
   const int threads = 256;
-  const int waves = std::max(1, std::min(nfw, 4 * m->props.cus));
+  const int waves = std::max(1, std::min(nfw, kAsyncWavesPerCu * m->props.cus));
   const int blocks = (waves + threads / 64 - 1) / (threads / 64);
   const unsigned long long max_ticks = fixpoint_budget_ticks(m);
   DevNet net = m->dm.net();
