@@ -58,9 +58,11 @@ __global__ void k_build_planes(const u64* __restrict__ rel, int nb, int max_dom,
 // Termination uses epochs that only grow (last_push_round, wipe_round), so
 // every block takes the same decision after a barrier without any reset.
 // ===========================================================================
-// Statistics counters are spread over kStatSlots addresses (by block): one
-// address taking an atomic from every wave of a large grid serializes.
-constexpr int kStatSlots = 32;
+// Statistics counters are spread over kStatSlots addresses (by block), and
+// the grid rounds add one workgroup total per round: device-scope atomics on
+// one address serialize, and one from every wave of an 8,192-wave round on
+// 32 addresses took 80 of the 100 us of a C2 round.
+constexpr int kStatSlots = 256;
 
 struct GacCtl {
   unsigned scanner[3];
@@ -222,9 +224,12 @@ __device__ __forceinline__ void grid_pairs_w1(const DevNet& net, u64* dom, int* 
 // loads (rounds of one launch, written by other XCDs since the last barrier).
 // With one-word domains, binary constraints go kPairBatch at a time through
 // grid_pairs_w1.
+// blk: the workgroup's two statistics totals in LDS (revisions, deletions),
+// zero on entry and left zero; every thread of the workgroup calls this.
 template <bool kCoherent = false>
 __device__ __forceinline__ void grid_round(const DevNet& net, u64* dom, int* size, unsigned* F,
-                                           int nfw, GacCtl* ctl, int r, int parts) {
+                                           int nfw, GacCtl* ctl, int r, int parts,
+                                           unsigned long long* blk) {
   unsigned* A = F + (long long)(r % 3) * nfw;
   unsigned* B = F + (long long)((r + 1) % 3) * nfw;
   unsigned* Cl = F + (long long)((r + 2) % 3) * nfw;
@@ -277,15 +282,30 @@ __device__ __forceinline__ void grid_round(const DevNet& net, u64* dom, int* siz
     }
   }
   if (lane == 0 && (revs | dels)) {
-    atomicAdd(&ctl->revisions[blockIdx.x % kStatSlots], revs);
-    atomicAdd(&ctl->deletions[blockIdx.x % kStatSlots], dels);
+    atomicAdd(&blk[0], revs);
+    atomicAdd(&blk[1], dels);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (blk[0] | blk[1]) {
+      atomicAdd(&ctl->revisions[blockIdx.x % kStatSlots], blk[0]);
+      atomicAdd(&ctl->deletions[blockIdx.x % kStatSlots], blk[1]);
+    }
+    blk[0] = 0ull;
+    blk[1] = 0ull;
   }
 }
 
 // One round per launch (host-driven fixpoint).
 __global__ void k_gac_round(DevNet net, u64* dom, int* size, unsigned* F, int nfw, GacCtl* ctl,
                             int r, int parts) {
-  grid_round(net, dom, size, F, nfw, ctl, r, parts);
+  __shared__ unsigned long long blk[2];
+  if (threadIdx.x == 0) {
+    blk[0] = 0ull;
+    blk[1] = 0ull;
+  }
+  __syncthreads();
+  grid_round(net, dom, size, F, nfw, ctl, r, parts, blk);
 }
 
 // ===========================================================================
@@ -370,11 +390,17 @@ __global__ void __launch_bounds__(kPersistentThreads) k_gac_persistent(DevNet ne
                                                         GacCtl* ctl, int max_rounds, int parts,
                                                         unsigned long long max_ticks) {
   __shared__ int s_decision;
+  __shared__ unsigned long long blk[2];
+  if (threadIdx.x == 0) {
+    blk[0] = 0ull;
+    blk[1] = 0ull;
+  }
+  __syncthreads();
   const unsigned long long t0 = wall_clock64();
   int r = 0;
   int decision = kRoundNext;
   for (; r < max_rounds; ++r) {
-    grid_round<true>(net, dom, size, F, nfw, ctl, r, parts);
+    grid_round<true>(net, dom, size, F, nfw, ctl, r, parts, blk);
     decision = grid_barrier_decide(ctl, gridDim.x, r, t0, max_ticks, &s_decision);
     if (decision != kRoundNext) break;
   }
@@ -400,9 +426,9 @@ __global__ void k_gac_init(GacCtl* ctl, unsigned* F, int nfw, int ncons, int see
     ctl->err = 0;
     ctl->pad2 = 0;
   }
-  if (i0 < kStatSlots) {
-    ctl->deletions[i0] = 0ull;
-    ctl->revisions[i0] = 0ull;
+  for (long long i = i0; i < kStatSlots; i += (long long)gridDim.x * blockDim.x) {
+    ctl->deletions[i] = 0ull;
+    ctl->revisions[i] = 0ull;
   }
   for (long long i = i0; i < 3ll * nfw; i += (long long)gridDim.x * blockDim.x) {
     unsigned v = 0u;

@@ -171,7 +171,7 @@ int cpim_gmodel_enforce_gac(cpim_gmodel* model, int32_t assigned_var, int32_t mo
  * cpim_gmodel_enforce_gac for callers that cannot allocate (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc): it
  * allocates nothing.  Its scratch (control block and frontier bitmaps) is a
- * file-scope device array of 769 KiB per GPU, held until the process ends, so
+ * file-scope device array of 776 KiB per GPU, held until the process ends, so
  * it takes networks of at most 2,097,152 propagated constraints
  * (CPIM_ERR_INVALID_ARGUMENT above that).  Calls that overlap on one GPU race
  * on that array.  stats->kernel_ms is 0 (no events are created). */
