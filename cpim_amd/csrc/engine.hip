@@ -1321,9 +1321,9 @@ void run_singleton_batched(cpim_gmodel* m, const int2* tests_host, long long cou
     S.fails = DevAlloc(count);
     S.cap = count;
   }
-  if (!S.revs.p) S.revs = DevAlloc(sizeof(unsigned long long));
+  if (!S.revs.p) S.revs = DevAlloc(sizeof(unsigned long long) * kStatSlots);  // per-slot counters
   HIPC(hipMemcpyAsync(S.tests.p, tests_host, sizeof(int2) * count, hipMemcpyHostToDevice, m->stream));
-  HIPC(hipMemsetAsync(S.revs.p, 0, sizeof(unsigned long long), m->stream));
+  HIPC(hipMemsetAsync(S.revs.p, 0, sizeof(unsigned long long) * kStatSlots, m->stream));
   const int L = m->current_level;
   DevNet net = m->dm.net();
   const long long bytes = wg_mem_bytes(m->dm.n, m->dm.W);
@@ -1350,9 +1350,10 @@ void run_singleton_batched(cpim_gmodel* m, const int2* tests_host, long long cou
   }
   HIPC(hipEventRecord(S.ev[1], m->stream));
   HIPC(hipMemcpyAsync(fail_host, S.fails.p, count, hipMemcpyDeviceToHost, m->stream));
-  unsigned long long revs = 0;
-  HIPC(hipMemcpyAsync(&revs, S.revs.p, sizeof(revs), hipMemcpyDeviceToHost, m->stream));
+  unsigned long long revs_slots[kStatSlots];
+  HIPC(hipMemcpyAsync(revs_slots, S.revs.p, sizeof(revs_slots), hipMemcpyDeviceToHost, m->stream));
   m->sync();
+  const unsigned long long revs = slot_sum(revs_slots);
   float ms = 0.f;
   HIPC(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
   st->kernel_ms += ms;
@@ -1591,13 +1592,13 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
     S.gfirst = DevAlloc(sizeof(int) * (groups + 1));
     S.gcap = groups + 1;
   }
-  if (!S.revs.p) S.revs = DevAlloc(sizeof(unsigned long long));
+  if (!S.revs.p) S.revs = DevAlloc(sizeof(unsigned long long) * kStatSlots);  // per-slot counters
   S.events();
   HIPC(hipMemcpyAsync(S.stests.p, sorted, sizeof(int2) * count, hipMemcpyHostToDevice, m->stream));
   if (!sorted_in)
     HIPC(hipMemcpyAsync(S.perm.p, order.data(), sizeof(int) * count, hipMemcpyHostToDevice, m->stream));
   HIPC(hipMemcpyAsync(S.gfirst.p, gfirst.data(), sizeof(int) * (groups + 1), hipMemcpyHostToDevice, m->stream));
-  HIPC(hipMemsetAsync(S.revs.p, 0, sizeof(unsigned long long), m->stream));
+  HIPC(hipMemsetAsync(S.revs.p, 0, sizeof(unsigned long long) * kStatSlots, m->stream));
   const long long slice_bytes = 8ll * m->dm.n * m->dm.dpad * M;
   const long long per_launch = std::max<long long>(1, std::min<long long>(groups, kSliceBudgetBytes / slice_bytes));
   u64* slices = nullptr;
@@ -1654,9 +1655,10 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
   }
   HIPC(hipEventRecord(S.ev[1], m->stream));
   HIPC(hipMemcpyAsync(fail_host, S.fails.p, count, hipMemcpyDeviceToHost, m->stream));
-  unsigned long long revs = 0;
-  HIPC(hipMemcpyAsync(&revs, S.revs.p, sizeof(revs), hipMemcpyDeviceToHost, m->stream));
+  unsigned long long revs_slots[kStatSlots];
+  HIPC(hipMemcpyAsync(revs_slots, S.revs.p, sizeof(revs_slots), hipMemcpyDeviceToHost, m->stream));
   m->sync();
+  const unsigned long long revs = slot_sum(revs_slots);
   float ms = 0.f;
   HIPC(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
   st->kernel_ms += ms;

@@ -1071,7 +1071,7 @@ __global__ void k_singleton_tests(DevNet net, const u64* __restrict__ base_dom,
   const bool ok = wg_fixpoint(net, mem, &ctl, false);
   if (threadIdx.x == 0) {
     fail[unit] = ok ? 0 : 1;
-    if (ctl.revs) atomicAdd(revs_out, ctl.revs);
+    if (ctl.revs) atomicAdd(&revs_out[blockIdx.x % kStatSlots], ctl.revs);
   }
 }
 
@@ -1389,7 +1389,7 @@ __global__ void k_sliced_tests(DevNet net, const u64* __restrict__ base_dom,
   if (lane == 0 && revs) atomicAdd(&ctl.revs, revs);
   __syncthreads();
   if (tid < tc) fail[perm ? perm[tf + tid] : tf + tid] = ((ctl.act >> tid) & 1ull) ? 0 : 1;
-  if (tid == 0 && ctl.revs) atomicAdd(revs_out, ctl.revs);
+  if (tid == 0 && ctl.revs) atomicAdd(&revs_out[blockIdx.x % kStatSlots], ctl.revs);
 }
 
 // ---------------------------------------------------------------------------
@@ -1907,7 +1907,7 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
   __syncthreads();
   if (tid < tc)
     fail[perm ? perm[tf + tid] : tf + tid] = ((ctl.act[tid >> 6] >> (tid & 63)) & 1ull) ? 0 : 1;
-  if (tid == 0 && ctl.revs) atomicAdd(revs_out, ctl.revs);
+  if (tid == 0 && ctl.revs) atomicAdd(&revs_out[blockIdx.x % kStatSlots], ctl.revs);
   // leave the region all zero: clear the slices of every variable written
   for (int v = wave; v < n; v += nwaves) {
     if (!uniform_int((int)bit_of(touched, v))) continue;
