@@ -73,8 +73,7 @@ struct GacCtl {
   int pad;
   // grid barrier of the persistent kernel: arrivals in this generation, the
   // generation, and the error flag set by a block whose wait ran out of time
-  unsigned bar_count;
-  unsigned bar_gen;
+  unsigned long long bar_ticket;  // grid barrier arrivals so far (only grows)
   int err;
   int pad2;
   unsigned long long deletions[kStatSlots];
@@ -313,9 +312,9 @@ __global__ void k_gac_round(DevNet net, u64* dom, int* size, unsigned* F, int nf
 // barrier.  The grid is sized by the host so that every workgroup is resident
 // at once (occupancy x CUs).
 //
-// The barrier is a generation counter: thread 0 of each workgroup reads the
-// generation (acquire), then adds itself to the arrivals; the last arrival
-// resets the count and bumps the generation, the others wait for the bump.
+// The barrier is a ticket count that only grows: thread 0 of each workgroup
+// takes a ticket (an atomic add that returns the count) and waits until the
+// count reaches the end of its ticket's round, nblocks tickets per round.
 // Every wait is bounded: a waiter whose device wall clock passes the launch's
 // budget, or that sees the error flag another block raised, stops waiting and
 // its block leaves, so the launch always ends (an error, never a hang), even
@@ -331,12 +330,16 @@ enum : int { kRoundNext = 0, kRoundWipe = 1, kRoundFixpoint = 2, kRoundError = 3
 
 // Every write a round makes that another workgroup reads in a later round
 // (domain words, sizes, frontier bits, epochs, the cleared frontier) is an
-// agent-scope atomic, and every such read is an agent-scope atomic load.  So
-// before the barrier each wave only waits until its own memory operations
-// have completed (s_waitcnt vmcnt(0): atomics with and without return, and
-// stores), and one thread per workgroup fences: a full release fence in every
-// wave (an L2 write-back each) made a barrier cost 37 us at 1,024 waves
-// against 12 us for a launch and sync.
+// agent-scope atomic, and every such read is an agent-scope atomic load, so
+// the data never waits in a cache that another XCD cannot see.  Before the
+// barrier each wave therefore only waits until its own memory operations have
+// completed (s_waitcnt vmcnt(0): atomics with and without return, and
+// stores), and the barrier's own accesses are relaxed: no fence, no L2
+// write-back or invalidation.  (A release fence in every wave made a barrier
+// cost 37 us at 1,024 waves; with one fence-pair per workgroup a 21-round
+// cascade took 1.41 ms, fence-free 1.06 ms.)  The
+// asynchronous fixpoint relies on the same coherence of agent-scope
+// accesses, with no barrier at all.
 __device__ __forceinline__ void wait_memory_ops() {
   // vmcnt(0); expcnt and lgkmcnt left at their maxima (no wait)
   __builtin_amdgcn_s_waitcnt(0x0F70);
@@ -348,29 +351,24 @@ __device__ __forceinline__ int grid_barrier_decide(GacCtl* ctl, unsigned nblocks
   wait_memory_ops();
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();  // release: the workgroup's completed round
-    const unsigned gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned arrived = atomicAdd(&ctl->bar_count, 1u) + 1u;
+    // every workgroup takes one ticket per round, so round r's tickets are
+    // r * nblocks .. (r + 1) * nblocks - 1; the round is complete once the
+    // count reaches the end of this ticket's round
+    const unsigned long long ticket = atomicAdd(&ctl->bar_ticket, 1ull);
+    const unsigned long long target = (ticket / nblocks + 1) * nblocks;
     int st = kRoundNext;
-    if (arrived == nblocks) {
-      atomicExch(&ctl->bar_count, 0u);
-      __threadfence();
-      atomicAdd(&ctl->bar_gen, 1u);
-    } else {
-      while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        if (__hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-          st = kRoundError;
-          break;
-        }
-        if (wall_clock64() - t0 > max_ticks) {
-          atomicExch(&ctl->err, 1);
-          st = kRoundError;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
+    while (__hip_atomic_load(&ctl->bar_ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        st = kRoundError;
+        break;
       }
+      if (wall_clock64() - t0 > max_ticks) {
+        atomicExch(&ctl->err, 1);
+        st = kRoundError;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
     }
-    __threadfence();  // acquire: every block's round
     if (st == kRoundNext && __hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
       st = kRoundError;
     if (st == kRoundNext) {
@@ -421,8 +419,7 @@ __global__ void k_gac_init(GacCtl* ctl, unsigned* F, int nfw, int ncons, int see
     ctl->rounds = 0;
     ctl->converged = 0;
     ctl->pad = 0;
-    ctl->bar_count = 0u;
-    ctl->bar_gen = 0u;
+    ctl->bar_ticket = 0ull;
     ctl->err = 0;
     ctl->pad2 = 0;
   }
