@@ -223,12 +223,11 @@ __device__ __forceinline__ void grid_pairs_w1(const DevNet& net, u64* dom, int* 
 // loads (rounds of one launch, written by other XCDs since the last barrier).
 // With one-word domains, binary constraints go kPairBatch at a time through
 // grid_pairs_w1.
-// blk: the workgroup's two statistics totals in LDS (revisions, deletions),
-// zero on entry and left zero; every thread of the workgroup calls this.
+// revs / dels: the calling wave's statistics, accumulated.
 template <bool kCoherent = false>
 __device__ __forceinline__ void grid_round(const DevNet& net, u64* dom, int* size, unsigned* F,
                                            int nfw, GacCtl* ctl, int r, int parts,
-                                           unsigned long long* blk) {
+                                           unsigned long long& revs, unsigned long long& dels) {
   unsigned* A = F + (long long)(r % 3) * nfw;
   unsigned* B = F + (long long)((r + 1) % 3) * nfw;
   unsigned* Cl = F + (long long)((r + 2) % 3) * nfw;
@@ -238,8 +237,6 @@ __device__ __forceinline__ void grid_round(const DevNet& net, u64* dom, int* siz
     if (kCoherent) __hip_atomic_store(&Cl[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else Cl[i] = 0u;
   }
-  const int lane = lane_id();
-  unsigned long long revs = 0, dels = 0;
   const long long units = (long long)nfw * parts;
   const int pbits = 32 / parts;
   // static stride over the units: one counter for tens of thousands of units
@@ -280,18 +277,21 @@ __device__ __forceinline__ void grid_round(const DevNet& net, u64* dom, int* siz
       grid_process_constraint<kCoherent>(net, dom, size, wi * 32 + b, B, ctl, r, revs, dels);
     }
   }
-  if (lane == 0 && (revs | dels)) {
+}
+
+// Adds the waves' statistics to the grid's totals: one workgroup total per
+// call, on the workgroup's slot.  blk: two LDS words, zero on entry.  Every
+// thread of the workgroup calls it.
+__device__ __forceinline__ void flush_grid_stats(GacCtl* ctl, unsigned long long* blk, unsigned long long revs,
+                                                 unsigned long long dels) {
+  if (lane_id() == 0 && (revs | dels)) {
     atomicAdd(&blk[0], revs);
     atomicAdd(&blk[1], dels);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    if (blk[0] | blk[1]) {
-      atomicAdd(&ctl->revisions[blockIdx.x % kStatSlots], blk[0]);
-      atomicAdd(&ctl->deletions[blockIdx.x % kStatSlots], blk[1]);
-    }
-    blk[0] = 0ull;
-    blk[1] = 0ull;
+  if (threadIdx.x == 0 && (blk[0] | blk[1])) {
+    atomicAdd(&ctl->revisions[blockIdx.x % kStatSlots], blk[0]);
+    atomicAdd(&ctl->deletions[blockIdx.x % kStatSlots], blk[1]);
   }
 }
 
@@ -304,7 +304,9 @@ __global__ void k_gac_round(DevNet net, u64* dom, int* size, unsigned* F, int nf
     blk[1] = 0ull;
   }
   __syncthreads();
-  grid_round(net, dom, size, F, nfw, ctl, r, parts, blk);
+  unsigned long long revs = 0, dels = 0;
+  grid_round(net, dom, size, F, nfw, ctl, r, parts, revs, dels);
+  flush_grid_stats(ctl, blk, revs, dels);
 }
 
 // ===========================================================================
@@ -397,11 +399,13 @@ __global__ void __launch_bounds__(kPersistentThreads) k_gac_persistent(DevNet ne
   const unsigned long long t0 = wall_clock64();
   int r = 0;
   int decision = kRoundNext;
+  unsigned long long revs = 0, dels = 0;  // the wave's, over every round
   for (; r < max_rounds; ++r) {
-    grid_round<true>(net, dom, size, F, nfw, ctl, r, parts, blk);
+    grid_round<true>(net, dom, size, F, nfw, ctl, r, parts, revs, dels);
     decision = grid_barrier_decide(ctl, gridDim.x, r, t0, max_ticks, &s_decision);
     if (decision != kRoundNext) break;
   }
+  flush_grid_stats(ctl, blk, revs, dels);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->rounds = r < max_rounds ? r + 1 : max_rounds;
     ctl->converged = decision == kRoundFixpoint ? 1 : 0;
