@@ -1437,12 +1437,14 @@ __host__ __device__ inline long long wide_stage_words(int W, int M, bool nib, in
 // Stage buffers: two (double buffered) without byte tables, one with them.
 __host__ __device__ inline int wide_stage_buffers(int nb) { return nb > 0 ? 1 : 2; }
 
-// LDS bytes: list [n], cur/nxt/touched bitmaps, the stage buffers and as many
-// alive-value rows of W words.
+// LDS bytes: list [n], cur/nxt/touched bitmaps, up to 255 bytes that align
+// the stage buffers to 256 bytes, the stage buffers and as many alive-value
+// rows of W words.  Every nibble-table buffer is a multiple of 256 bytes, so
+// each buffer and each half-word table in it stays aligned.
 __host__ __device__ inline long long wide_lds_bytes(int n, int W, int M, bool nib = false, int nb = 0) {
   const long long nvw = (n + 31) / 32;
   long long b = 4ll * n + 12ll * nvw;
-  b = (b + 15) & ~15ll;
+  b = ((b + 15) & ~15ll) + 256;
   const int nbuf = wide_stage_buffers(nb);
   b += nbuf * 8ll * wide_stage_words(W, M, nib, nb) + nbuf * 8ll * W;
   return b;
@@ -1504,8 +1506,88 @@ __device__ __forceinline__ int nib_word(int q, int sub, int mm) {
   else return (q * 16 + sub) * M + mm;
 }
 
+// p & ~(a | b | c | d): one three-input OR and one gfx950 three-input logic
+// op (truth table 0x10: first operand set, the other two clear).
+__device__ __forceinline__ unsigned andnot_or4(unsigned p, unsigned a, unsigned b, unsigned c, unsigned d) {
+  return __builtin_amdgcn_bitop3_b32(p, a | b | c, d, 0x10);
+}
+
+// (a ^ b) & c, one three-input op per 32-bit half (truth table 0x28).
+__device__ __forceinline__ u64 xor_and64(u64 a, u64 b, u64 c) {
+  const unsigned lo = __builtin_amdgcn_bitop3_b32((unsigned)a, (unsigned)b, (unsigned)c, 0x28);
+  const unsigned hi = __builtin_amdgcn_bitop3_b32((unsigned)(a >> 32), (unsigned)(b >> 32), (unsigned)(c >> 32), 0x28);
+  return (u64)lo | ((u64)hi << 32);
+}
+
+// a & b as one op per 32-bit half with b uniform (a select of b would first
+// copy b out of its scalar register).
+__device__ __forceinline__ u64 and64(u64 a, u64 b) {
+  const unsigned lo = __builtin_amdgcn_bitop3_b32((unsigned)a, (unsigned)b, 0u, 0xC0);
+  const unsigned hi = __builtin_amdgcn_bitop3_b32((unsigned)(a >> 32), (unsigned)(b >> 32), 0u, 0xC0);
+  return (u64)lo | ((u64)hi << 32);
+}
+
+// 32-bit LDS byte address of a pointer into shared memory.
+__device__ __forceinline__ unsigned lds_addr(const void* p) {
+  return (unsigned)(size_t)(__attribute__((address_space(3))) const char*)p;
+}
+
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const v4u lds_v4u;
+
+// Four masks, nibbles from 0: the same lookups with the table addresses and
+// the OR-and-clear written for the gfx950 ALU.  tab must be 256-byte aligned
+// in LDS, so that the subset offset (16 bytes per subset) is ORed into the
+// base with one three-input op ((x & 0xF0) | base, truth table 0xEA) and the
+// nibble's block offset rides in the read's immediate.  Per step of four
+// lookups: 7 address ops and 16 mask ops, where the generic form takes 11
+// and 24.
+__device__ __forceinline__ void clear_supported_nib4(u64 (&pend)[4], unsigned bits, const u64* tab) {
+  const unsigned base = lds_addr(tab);
+  unsigned p[8];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    p[2 * m] = (unsigned)pend[m];
+    p[2 * m + 1] = (unsigned)(pend[m] >> 32);
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q += 4) {
+    if (!bits || !((p[0] | p[1] | p[2] | p[3]) | (p[4] | p[5] | p[6] | p[7]))) break;
+    v4u lo[4], hi[4];
+    unsigned a[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int sh = 4 * (q + i) - 4;
+      const unsigned x = sh >= 0 ? (bits >> sh) : (bits << 4);
+      a[i] = __builtin_amdgcn_bitop3_b32(x, 0xF0u, base, 0xEA) + (q + i) * 512;
+      lo[i] = ((lds_v4u*)(size_t)a[i])[0];  // masks 0, 1 of the subset
+    }
+    p[0] = andnot_or4(p[0], lo[0].x, lo[1].x, lo[2].x, lo[3].x);
+    p[1] = andnot_or4(p[1], lo[0].y, lo[1].y, lo[2].y, lo[3].y);
+    p[2] = andnot_or4(p[2], lo[0].z, lo[1].z, lo[2].z, lo[3].z);
+    p[3] = andnot_or4(p[3], lo[0].w, lo[1].w, lo[2].w, lo[3].w);
+    // the second plane's reads after the first plane's ORs: eight reads in
+    // flight at once would hold 32 registers and spill at the 64-register
+    // budget
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) hi[i] = ((lds_v4u*)(size_t)a[i])[16];  // masks 2, 3, 256 bytes on
+    p[4] = andnot_or4(p[4], hi[0].x, hi[1].x, hi[2].x, hi[3].x);
+    p[5] = andnot_or4(p[5], hi[0].y, hi[1].y, hi[2].y, hi[3].y);
+    p[6] = andnot_or4(p[6], hi[0].z, hi[1].z, hi[2].z, hi[3].z);
+    p[7] = andnot_or4(p[7], hi[0].w, hi[1].w, hi[2].w, hi[3].w);
+    bits &= ~(0xffffu << (4 * q));
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) pend[m] = (u64)p[2 * m] | ((u64)p[2 * m + 1] << 32);
+}
+
 template <int M, int Q0 = 0>
 __device__ __forceinline__ void clear_supported_nib(u64 (&pend)[M], unsigned bits, const u64* tab) {
+  if constexpr (M == 4 && Q0 == 0 && kNibStep == 4) {
+    clear_supported_nib4(pend, bits, tab);
+    return;
+  }
   for (int q = Q0; q < 8 && bits && any_mask<M>(pend); q += kNibStep) {
     u64 acc[M];
 #pragma unroll
@@ -1599,6 +1681,10 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
                                             const u64 (&all)[M]) {
   const int lane = lane_id();
   bool changed = false;
+  u64 aa[M];  // live tests of the group
+#pragma unroll
+  for (int m = 0; m < M; ++m) aa[m] = all[m] & act[m];
+  const u64 atw0 = atw[0];
   // word 0 of the next chunk's rows is loaded one chunk ahead, so that its
   // latency overlaps this chunk's lookups
   u64 row_next = plane_u[lane];
@@ -1618,14 +1704,14 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
       u64 sv[M];
       load_masks<M>(&slu[(long long)b * M], sv);
 #pragma unroll
-      for (int m = 0; m < M; ++m) pend[m] = all[m] & act[m] & (sv[m] ^ flip);
+      for (int m = 0; m < M; ++m) pend[m] = xor_and64(sv[m], flip, aa[m]);
     } else {
 #pragma unroll
-      for (int m = 0; m < M; ++m) pend[m] = in ? (all[m] & act[m]) : 0ull;
+      for (int m = 0; m < M; ++m) pend[m] = and64(flip, aa[m]);
     }
     const u64* nib = nib_base<M, NB>(stage);
     if (any_mask<M>(pend)) {
-      const u64 r = row0 & atw[0];
+      const u64 r = row0 & atw0;
       if (NB > 0) {
         const u64 rest = clear_supported_bytes<M, NB>(pend, r, stage);
         if (NB < 4) clear_supported_nib<M, 2 * NB>(pend, (unsigned)rest, nib);
@@ -1707,7 +1793,11 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
   unsigned* cur = (unsigned*)(list + n);
   unsigned* nxt = cur + nvw;
   unsigned* touched = nxt + nvw;
-  const long long head = (4ll * n + 12ll * nvw + 15) & ~15ll;
+  // the stage buffers start 256-byte aligned in LDS: the four-mask nibble
+  // lookups OR the subset offset into the table base
+  const unsigned smem0 = lds_addr(smem);
+  const unsigned head_raw = (unsigned)((4ll * n + 12ll * nvw + 15) & ~15ll);
+  const long long head = (long long)(((smem0 + head_raw + 255u) & ~255u) - smem0);
   u64* stage0 = (u64*)(smem + head);
   static_assert(NB == 0 || (kNib && NB <= 6), "byte tables go with nibble tables");
   const long long stage_words = wide_stage_words(W, M, kNib, NB);
