@@ -1541,7 +1541,11 @@ typedef __attribute__((address_space(3))) const v4u lds_v4u;
 // base with one three-input op ((x & 0xF0) | base, truth table 0xEA) and the
 // nibble's block offset rides in the read's immediate.  Per step of four
 // lookups: 7 address ops and 16 mask ops, where the generic form takes 11
-// and 24.
+// and 24.  Without kCheck both steps run without the early-exit tests (for
+// the first half-word of a chunk, where some test of nearly every lane still
+// lacks a support after 16 values: the tests cost 12 ops per chunk, and a
+// lane with no bits left only looks up the zero subset).
+template <bool kCheck>
 __device__ __forceinline__ void clear_supported_nib4(u64 (&pend)[4], unsigned bits, const u64* tab) {
   const unsigned base = lds_addr(tab);
   unsigned p[8];
@@ -1552,7 +1556,7 @@ __device__ __forceinline__ void clear_supported_nib4(u64 (&pend)[4], unsigned bi
   }
 #pragma unroll
   for (int q = 0; q < 8; q += 4) {
-    if (!bits || !((p[0] | p[1] | p[2] | p[3]) | (p[4] | p[5] | p[6] | p[7]))) break;
+    if (kCheck && (!bits || !((p[0] | p[1] | p[2] | p[3]) | (p[4] | p[5] | p[6] | p[7])))) break;
     v4u lo[4], hi[4];
     unsigned a[4];
 #pragma unroll
@@ -1582,10 +1586,10 @@ __device__ __forceinline__ void clear_supported_nib4(u64 (&pend)[4], unsigned bi
   for (int m = 0; m < 4; ++m) pend[m] = (u64)p[2 * m] | ((u64)p[2 * m + 1] << 32);
 }
 
-template <int M, int Q0 = 0>
+template <int M, int Q0 = 0, bool kCheck = true>
 __device__ __forceinline__ void clear_supported_nib(u64 (&pend)[M], unsigned bits, const u64* tab) {
   if constexpr (M == 4 && Q0 == 0 && kNibStep == 4) {
-    clear_supported_nib4(pend, bits, tab);
+    clear_supported_nib4<kCheck>(pend, bits, tab);
     return;
   }
   for (int q = Q0; q < 8 && bits && any_mask<M>(pend); q += kNibStep) {
@@ -1717,7 +1721,7 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
         if (NB < 4) clear_supported_nib<M, 2 * NB>(pend, (unsigned)rest, nib);
         clear_supported_nib<M, (NB > 4 ? 2 * NB - 8 : 0)>(pend, (unsigned)(rest >> 32), nib + 128ll * M);
       } else if (kNib) {
-        clear_supported_nib<M>(pend, (unsigned)r, stage);
+        clear_supported_nib<M, 0, false>(pend, (unsigned)r, stage);
         clear_supported_nib<M>(pend, (unsigned)(r >> 32), stage + 128ll * M);
       } else {
         clear_supported_wide<M>(pend, (unsigned)r, stage + (long long)1 * M);
