@@ -1826,15 +1826,29 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
   }
   __syncthreads();
   // test k reduces its variable x_k to {a_k}: flip every base value but a_k,
-  // and a_k if it is not a base value
-  for (int k = wave; k < tc; k += nwaves) {
-    const int2 tk = tests[tf + k];
-    const int m = k >> 6;
-    const u64 bit = 1ull << (k & 63);
-    for (int b = lane; b < dpad; b += 64) {
-      const bool in = (base_dom[(long long)tk.x * W + (b >> 6)] >> (b & 63)) & 1ull;
-      if (in != (b == tk.y)) or_wg(&sl[((long long)tk.x * dpad + b) * M + m], bit);
+  // and a_k if it is not a base value.  The tests of a group come in runs of
+  // one variable (the list is sorted by variable), so word (x, b, m) of the
+  // slices is built whole by one thread from the 64 tests of mask m and
+  // stored once: no atomics (one per flipped value and test, 64 to a word,
+  // took a sixth of the pass).
+  for (int w = tid; w < dpad * M; w += nthreads) {
+    const int m = (w >> 6) % M;  // uniform: nthreads is a multiple of 64
+    const int b = ((w >> 6) / M) * 64 + (w & 63);
+    const int kend = min(tc, 64 * (m + 1));
+    int x = -1;
+    bool in = false;
+    u64 word = 0ull;
+    for (int k = 64 * m; k < kend; ++k) {
+      const int2 tk = tests[tf + k];
+      if (tk.x != x) {
+        if (word) sl[((long long)x * dpad + b) * M + m] = word;
+        x = tk.x;
+        word = 0ull;
+        in = (base_dom[(long long)x * W + (b >> 6)] >> (b & 63)) & 1ull;
+      }
+      if (in != (b == tk.y)) word |= 1ull << (k & 63);
     }
+    if (word) sl[((long long)x * dpad + b) * M + m] = word;
   }
   if (tid < tc) {
     const int x = tests[tf + tid].x;
