@@ -1943,7 +1943,11 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
           }
         }
       }
-      for (int v = tid; v < dpad; v += nthreads) {
+      // the alive-any words from the waves after the table items when there
+      // are enough of them, so that the two sets of slice reads overlap
+      const int items = kNib ? (16 * W - 2 * NB) * M + NB * M * 16 : 0;
+      const int abase = ((items + 63) & ~63) + dpad <= nthreads ? ((items + 63) & ~63) : 0;
+      for (int v = tid - abase; tid >= abase && v < dpad; v += nthreads) {
         u64 alive = 0ull;
         u64* dst = stage + (long long)stage_pos(v) * M;
 #pragma unroll
@@ -1986,16 +1990,20 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
     for (int m = 0; m < M; ++m) alive_all[m] = ~0ull;
     for (int v = wave; v < n; v += nwaves) {
       if (!uniform_int((int)bit_of(nxt, v))) continue;
+      // a lane reads the M masks of its values together, every read of the
+      // variable in flight at once
+      u64 o[M];
 #pragma unroll
-      for (int m = 0; m < M; ++m) {
-        u64 o = 0ull;
-        for (int b = lane; b < dpad; b += 64) {
-          const u64 flip = ((base_dom[(long long)v * W + (b >> 6)] >> (b & 63)) & 1ull) ? ~0ull : 0ull;
-          o |= all[m] & (ld_wg(&sl[((long long)v * dpad + b) * M + m]) ^ flip);
-        }
-        o = wave_or64_dpp(o);
-        alive_all[m] &= o;
+      for (int m = 0; m < M; ++m) o[m] = 0ull;
+      for (int b = lane; b < dpad; b += 64) {
+        const u64 flip = ((base_dom[(long long)v * W + (b >> 6)] >> (b & 63)) & 1ull) ? ~0ull : 0ull;
+        u64 sv[M];
+        load_masks<M>(&sl[((long long)v * dpad + b) * M], sv);
+#pragma unroll
+        for (int m = 0; m < M; ++m) o[m] |= all[m] & (sv[m] ^ flip);
       }
+#pragma unroll
+      for (int m = 0; m < M; ++m) alive_all[m] &= wave_or64_dpp(o[m]);
     }
     if (lane == 0) {
 #pragma unroll
