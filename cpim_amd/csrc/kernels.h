@@ -1892,12 +1892,23 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
       // stage t: thread v holds value v's M masks; one wave covers one word
       const u64* slt = sl + (long long)t * dpad * M;
       const u64* base_t = base_dom + (long long)t * W;
-      // masks of value v of t: the live tests that still hold it
+      // masks of value v of t: the live tests that still hold it.  Plain
+      // loads, issued together: a wave still revising the previous
+      // variable's arcs may remove values of t meanwhile, and a stage that
+      // misses such a removal only keeps a support too many; t is then
+      // marked changed and staged again next round.
       auto alive_of = [&](int v, int mm) -> u64 {
         const u64 flip = ((base_t[v >> 6] >> (v & 63)) & 1ull) ? ~0ull : 0ull;
         const int kk = tc - 64 * mm;
         const u64 all_m = kk >= 64 ? ~0ull : (kk > 0 ? ((1ull << kk) - 1ull) : 0ull);
-        return all_m & (ld_wg(&slt[(long long)v * M + mm]) ^ flip);
+        return all_m & (slt[(long long)v * M + mm] ^ flip);
+      };
+      // the same from a slice word already loaded
+      auto alive_bits = [&](u64 raw, int v, int mm) -> u64 {
+        const u64 flip = ((base_t[v >> 6] >> (v & 63)) & 1ull) ? ~0ull : 0ull;
+        const int kk = tc - 64 * mm;
+        const u64 all_m = kk >= 64 ? ~0ull : (kk > 0 ? ((1ull << kk) - 1ull) : 0ull);
+        return all_m & (raw ^ flip);
       };
       // every wave finished the arcs of this buffer's previous variable
       // before the last barrier
@@ -1913,7 +1924,9 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
             const int q = 2 * NB + it / M, mm = it % M;
             u64 x[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) x[i] = alive_of(4 * q + i, mm);
+            for (int i = 0; i < 4; ++i) x[i] = slt[(long long)(4 * q + i) * M + mm];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) x[i] = alive_bits(x[i], 4 * q + i, mm);
 #pragma unroll
             for (int sub = 0; sub < 16; ++sub) {
               u64 o = 0ull;
@@ -1950,9 +1963,12 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
       for (int v = tid - abase; tid >= abase && v < dpad; v += nthreads) {
         u64 alive = 0ull;
         u64* dst = stage + (long long)stage_pos(v) * M;
+        u64 sv[M];
+        load_masks<M>(&slt[(long long)v * M], sv);
+        const u64 flip = ((base_t[v >> 6] >> (v & 63)) & 1ull) ? ~0ull : 0ull;
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-          const u64 x = alive_of(v, m);
+          const u64 x = all[m] & (sv[m] ^ flip);
           if (!kNib) dst[m] = x;
           alive |= x & act[m];
         }
