@@ -1688,7 +1688,9 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
   u64 aa[M];  // live tests of the group
 #pragma unroll
   for (int m = 0; m < M; ++m) aa[m] = all[m] & act[m];
+  const bool aa_any = any_mask<M>(aa);
   const u64 atw0 = atw[0];
+  const int lane_hi = lane & 32;
   // word 0 of the next chunk's rows is loaded one chunk ahead, so that its
   // latency overlaps this chunk's lookups
   u64 row_next = plane_u[lane];
@@ -1696,9 +1698,13 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
     const int b = j * 64 + lane;
     const u64 row0 = row_next;
     if (j + 1 < W) row_next = plane_u[b + 64];
-    const bool in = (base_u[j] >> lane) & 1ull;
-    const u64 flip = in ? ~0ull : 0ull;  // a set slice bit flips the base state
+    // flip: all ones when b is a base value (a set slice bit flips the base
+    // state), as one sign-extending bit-field extract
+    const int flip32 = __builtin_amdgcn_sbfe((int)(unsigned)(base_u[j] >> lane_hi), lane & 31, 1);
+    const bool in = flip32 != 0;
+    const u64 flip = (u64)(long long)flip32;
     u64 pend[M];
+    bool has;  // some test still holds b
     if (touched_u) {
       // plain loads, all M masks at once: another wave may be removing
       // values of u at the same time, and a load that misses those removals
@@ -1709,12 +1715,14 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
       load_masks<M>(&slu[(long long)b * M], sv);
 #pragma unroll
       for (int m = 0; m < M; ++m) pend[m] = xor_and64(sv[m], flip, aa[m]);
+      has = any_mask<M>(pend);
     } else {
 #pragma unroll
       for (int m = 0; m < M; ++m) pend[m] = and64(flip, aa[m]);
+      has = in && aa_any;
     }
     const u64* nib = nib_base<M, NB>(stage);
-    if (any_mask<M>(pend)) {
+    if (has) {
       const u64 r = row0 & atw0;
       if (NB > 0) {
         const u64 rest = clear_supported_bytes<M, NB>(pend, r, stage);
