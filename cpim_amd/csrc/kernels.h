@@ -1586,6 +1586,55 @@ __device__ __forceinline__ void clear_supported_nib4(u64 (&pend)[4], unsigned bi
   for (int m = 0; m < 4; ++m) pend[m] = (u64)p[2 * m] | ((u64)p[2 * m + 1] << 32);
 }
 
+__device__ __forceinline__ void split_masks(const u64 (&pend)[4], unsigned (&p)[8]) {
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    p[2 * m] = (unsigned)pend[m];
+    p[2 * m + 1] = (unsigned)(pend[m] >> 32);
+  }
+}
+
+__device__ __forceinline__ void join_masks(const unsigned (&p)[8], u64 (&pend)[4]) {
+#pragma unroll
+  for (int m = 0; m < 4; ++m) pend[m] = (u64)p[2 * m] | ((u64)p[2 * m + 1] << 32);
+}
+
+// The lookups of clear_supported_nib4 on 32-bit halves, with the caller's
+// any-pending flag live kept current: kCheck runs a step only while live and
+// bits remain, and recomputes live after each step; without kCheck both
+// steps run and live is computed once at the end.
+template <bool kCheck>
+__device__ __forceinline__ void nib4_run(unsigned (&p)[8], unsigned bits, const u64* tab, bool& live) {
+  const unsigned base = lds_addr(tab);
+#pragma unroll
+  for (int q = 0; q < 8; q += 4) {
+    if (kCheck && (!live || !bits)) break;
+    v4u lo[4], hi[4];
+    unsigned a[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int sh = 4 * (q + i) - 4;
+      const unsigned x = sh >= 0 ? (bits >> sh) : (bits << 4);
+      a[i] = __builtin_amdgcn_bitop3_b32(x, 0xF0u, base, 0xEA) + (q + i) * 512;
+      lo[i] = ((lds_v4u*)(size_t)a[i])[0];
+    }
+    p[0] = andnot_or4(p[0], lo[0].x, lo[1].x, lo[2].x, lo[3].x);
+    p[1] = andnot_or4(p[1], lo[0].y, lo[1].y, lo[2].y, lo[3].y);
+    p[2] = andnot_or4(p[2], lo[0].z, lo[1].z, lo[2].z, lo[3].z);
+    p[3] = andnot_or4(p[3], lo[0].w, lo[1].w, lo[2].w, lo[3].w);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) hi[i] = ((lds_v4u*)(size_t)a[i])[16];
+    p[4] = andnot_or4(p[4], hi[0].x, hi[1].x, hi[2].x, hi[3].x);
+    p[5] = andnot_or4(p[5], hi[0].y, hi[1].y, hi[2].y, hi[3].y);
+    p[6] = andnot_or4(p[6], hi[0].z, hi[1].z, hi[2].z, hi[3].z);
+    p[7] = andnot_or4(p[7], hi[0].w, hi[1].w, hi[2].w, hi[3].w);
+    bits &= ~(0xffffu << (4 * q));
+    if (kCheck) live = ((p[0] | p[1] | p[2] | p[3]) | (p[4] | p[5] | p[6] | p[7])) != 0u;
+  }
+  if (!kCheck) live = ((p[0] | p[1] | p[2] | p[3]) | (p[4] | p[5] | p[6] | p[7])) != 0u;
+}
+
 template <int M, int Q0 = 0, bool kCheck = true>
 __device__ __forceinline__ void clear_supported_nib(u64 (&pend)[M], unsigned bits, const u64* tab) {
   if constexpr (M == 4 && Q0 == 0 && kNibStep == 4) {
@@ -1722,7 +1771,25 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
       has = in && aa_any;
     }
     const u64* nib = nib_base<M, NB>(stage);
-    if (has) {
+    bool live = has;  // some lane still holds a test without a support
+    if constexpr (M == 4 && kNib && NB == 0) {
+      // the four-mask path keeps live current after every step, so that
+      // each step's test is computed once
+      if (has) {
+        unsigned p[8];
+        split_masks(pend, p);
+        const u64 r = row0 & atw0;
+        nib4_run<false>(p, (unsigned)r, stage, live);
+        nib4_run<true>(p, (unsigned)(r >> 32), stage + 128ll * M, live);
+        for (int w = 1; w < W && live; ++w) {
+          const u64 rw = plane_u[(long long)w * dpad + b] & atw[w];
+          const u64* tb = stage + 256ll * w * M;
+          nib4_run<true>(p, (unsigned)rw, tb, live);
+          nib4_run<true>(p, (unsigned)(rw >> 32), tb + 128ll * M, live);
+        }
+        join_masks(p, pend);
+      }
+    } else if (has) {
       const u64 r = row0 & atw0;
       if (NB > 0) {
         const u64 rest = clear_supported_bytes<M, NB>(pend, r, stage);
@@ -1736,19 +1803,22 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
         clear_supported_wide<M>(pend, (unsigned)(r >> 32), stage + (long long)34 * M);
       }
     }
-    for (int w = 1; w < W && any_mask<M>(pend); ++w) {
-      const u64 r = plane_u[(long long)w * dpad + b] & atw[w];
-      if (kNib) {
-        const u64* tb = nib + 256ll * w * M;
-        clear_supported_nib<M>(pend, (unsigned)r, tb);
-        clear_supported_nib<M>(pend, (unsigned)(r >> 32), tb + 128ll * M);
-      } else {
-        const u64* st = stage + (long long)w * kStageWord * M;
-        clear_supported_wide<M>(pend, (unsigned)r, st + (long long)1 * M);
-        clear_supported_wide<M>(pend, (unsigned)(r >> 32), st + (long long)34 * M);
+    if constexpr (!(M == 4 && kNib && NB == 0)) {
+      for (int w = 1; w < W && any_mask<M>(pend); ++w) {
+        const u64 r = plane_u[(long long)w * dpad + b] & atw[w];
+        if (kNib) {
+          const u64* tb = nib + 256ll * w * M;
+          clear_supported_nib<M>(pend, (unsigned)r, tb);
+          clear_supported_nib<M>(pend, (unsigned)(r >> 32), tb + 128ll * M);
+        } else {
+          const u64* st = stage + (long long)w * kStageWord * M;
+          clear_supported_wide<M>(pend, (unsigned)r, st + (long long)1 * M);
+          clear_supported_wide<M>(pend, (unsigned)(r >> 32), st + (long long)34 * M);
+        }
       }
+      live = any_mask<M>(pend);
     }
-    if (any_mask<M>(pend)) {
+    if (live) {
 #pragma unroll
       for (int m = 0; m < M; ++m) {
         if (pend[m]) {
