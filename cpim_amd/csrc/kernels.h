@@ -1600,22 +1600,25 @@ __device__ __forceinline__ void join_masks(const unsigned (&p)[8], u64 (&pend)[4
 }
 
 // The lookups of clear_supported_nib4 on 32-bit halves, with the caller's
-// any-pending flag live kept current: kCheck runs a step only while live and
-// bits remain, and recomputes live after each step; without kCheck both
-// steps run and live is computed once at the end.
+// any-pending flag live kept current: kCheck runs a step only while live
+// and recomputes live after each step; without kCheck both steps run and
+// live is computed once at the end.
 template <bool kCheck>
-__device__ __forceinline__ void nib4_run(unsigned (&p)[8], unsigned bits, const u64* tab, bool& live) {
+__device__ __forceinline__ void nib4_run(unsigned (&p)[8], unsigned bits, const u64* tab, bool& live,
+                                         unsigned vF0) {
   const unsigned base = lds_addr(tab);
 #pragma unroll
   for (int q = 0; q < 8; q += 4) {
-    if (kCheck && (!live || !bits)) break;
+    // a live lane with no bits left in this half only looks up zero
+    // subsets, so the wave tests live alone
+    if (kCheck && !live) break;
     v4u lo[4], hi[4];
     unsigned a[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int sh = 4 * (q + i) - 4;
       const unsigned x = sh >= 0 ? (bits >> sh) : (bits << 4);
-      a[i] = __builtin_amdgcn_bitop3_b32(x, 0xF0u, base, 0xEA) + (q + i) * 512;
+      a[i] = __builtin_amdgcn_bitop3_b32(x, vF0, base, 0xEA) + (q + i) * 512;
       lo[i] = ((lds_v4u*)(size_t)a[i])[0];
     }
     p[0] = andnot_or4(p[0], lo[0].x, lo[1].x, lo[2].x, lo[3].x);
@@ -1738,6 +1741,10 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
 #pragma unroll
   for (int m = 0; m < M; ++m) aa[m] = all[m] & act[m];
   const bool aa_any = any_mask<M>(aa);
+  // the subset mask of the table addresses in a vector register: a
+  // three-input op reads one scalar operand, and the table base is one
+  unsigned vF0;
+  asm volatile("v_mov_b32 %0, 0xf0" : "=v"(vF0));
   const u64 atw0 = atw[0];
   const int lane_hi = lane & 32;
   // word 0 of the next chunk's rows is loaded one chunk ahead, so that its
@@ -1779,13 +1786,13 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
         unsigned p[8];
         split_masks(pend, p);
         const u64 r = row0 & atw0;
-        nib4_run<false>(p, (unsigned)r, stage, live);
-        nib4_run<true>(p, (unsigned)(r >> 32), stage + 128ll * M, live);
+        nib4_run<false>(p, (unsigned)r, stage, live, vF0);
+        nib4_run<true>(p, (unsigned)(r >> 32), stage + 128ll * M, live, vF0);
         for (int w = 1; w < W && live; ++w) {
           const u64 rw = plane_u[(long long)w * dpad + b] & atw[w];
           const u64* tb = stage + 256ll * w * M;
-          nib4_run<true>(p, (unsigned)rw, tb, live);
-          nib4_run<true>(p, (unsigned)(rw >> 32), tb + 128ll * M, live);
+          nib4_run<true>(p, (unsigned)rw, tb, live, vF0);
+          nib4_run<true>(p, (unsigned)(rw >> 32), tb + 128ll * M, live, vF0);
         }
         join_masks(p, pend);
       }
