@@ -1773,8 +1773,10 @@ __device__ __forceinline__ bool wide_revise(const u64* __restrict__ plane_u, u64
       for (int m = 0; m < M; ++m) pend[m] = xor_and64(sv[m], flip, aa[m]);
       has = any_mask<M>(pend);
     } else {
+      // pend = aa for a base value, nothing otherwise; the four-mask path
+      // starts from aa itself, inside the lanes that hold b
 #pragma unroll
-      for (int m = 0; m < M; ++m) pend[m] = and64(flip, aa[m]);
+      for (int m = 0; m < M; ++m) pend[m] = (M == 4 && kNib && NB == 0) ? aa[m] : and64(flip, aa[m]);
       has = in && aa_any;
     }
     const u64* nib = nib_base<M, NB>(stage);
