@@ -560,7 +560,8 @@ long long now_ns() {
 struct SacScratch {
   DevAlloc tests, fails, revs;
   long long cap = 0;
-  DevAlloc stests, perm, gfirst;  // bit-sliced path: sorted tests, their order, group starts
+  DevAlloc stests, perm, gfirst, gorder;  // bit-sliced path: sorted tests, their order, group starts,
+                                          // group of each workgroup slot
   long long scap = 0, gcap = 0;
   // removal slices of the wide bit-sliced kernel: zeroed when allocated, and
   // left all zero by every launch of that kernel
@@ -1501,19 +1502,29 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
   REQUIRE(M > 0, "bit-sliced singleton tests do not fit in LDS");
   const long long cap = 64ll * M;  // tests per workgroup
   std::vector<int> gfirst;
+  std::vector<long long> w;  // expected work of each group: summed degree of its variables
   gfirst.reserve((size_t)(count / 32 + 2));
+  w.reserve((size_t)(count / 32 + 2));
   int open = -1;  // start of the group being filled, -1 if none
   for (long long i = 0; i < count;) {
     long long j = i;
     while (j < count && sorted[j].x == sorted[i].x) ++j;
     const long long len = j - i;
+    const int x = sorted[i].x;
+    const long long dx = m->im.adj_off[x + 1] - m->im.adj_off[x];
     if (len > cap) {
       open = -1;
       const long long k = (len + cap - 1) / cap;
-      for (long long p = 0; p < k; ++p) gfirst.push_back((int)(i + len * p / k));
+      for (long long p = 0; p < k; ++p) {
+        gfirst.push_back((int)(i + len * p / k));
+        w.push_back(dx);
+      }
     } else if (open < 0 || i + len - open > cap) {
       gfirst.push_back((int)i);
+      w.push_back(dx);
       open = (int)i;
+    } else {
+      w.back() += dx;
     }
     i = j;
   }
@@ -1526,17 +1537,11 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
   // workgroup, and every other heavy group shares its CU with a light one, so
   // that the kernel does not wait on two heavy groups sharing a CU.  With more
   // groups than slots, heavy groups go first.  (C3: 8.1 -> 7.1 ms per pass.)
+  // The kernel reads the group of each slot from gorder; the test list keeps
+  // its order.
+  std::vector<int> gorder;
   if (wide && groups > 1 && groups <= kSliceBudgetBytes / (8ll * m->dm.n * m->dm.dpad * M)) {
     const int cus = std::max(1, m->props.cus);
-    std::vector<long long> w((size_t)groups, 0);
-    for (long long g = 0; g < groups; ++g) {
-      int last = -1;
-      for (int i = gfirst[(size_t)g]; i < gfirst[(size_t)g + 1]; ++i) {
-        const int x = sorted[i].x;
-        if (x != last) w[(size_t)g] += m->im.adj_off[x + 1] - m->im.adj_off[x];
-        last = x;
-      }
-    }
     std::vector<int> rank((size_t)groups);
     for (long long g = 0; g < groups; ++g) rank[(size_t)g] = (int)g;
     std::stable_sort(rank.begin(), rank.end(), [&](int a, int b) { return w[(size_t)a] > w[(size_t)b]; });
@@ -1550,33 +1555,13 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
         at_bid[(size_t)(k + cus)] = rank[(size_t)(B - 1 - k)];
       }
     }
-    // the kernel takes group gi = xcd * q + min(xcd, rem) + (bid >> 3) for bid
+    // the kernel takes slot gi = xcd * q + min(xcd, rem) + (bid >> 3) for bid
     const int q = B >> 3, rem = B & 7;
-    std::vector<int> gorder((size_t)B);
+    gorder.resize((size_t)B);
     for (int bid = 0; bid < B; ++bid) {
       const int xcd = bid & 7;
       gorder[(size_t)(xcd * q + (xcd < rem ? xcd : rem) + (bid >> 3))] = at_bid[(size_t)bid];
     }
-    // rebuild the test list in that group order
-    std::vector<int2> re((size_t)count);
-    std::vector<int> reorder((size_t)count), nf;
-    nf.reserve((size_t)B + 1);
-    long long pos = 0;
-    for (int gi = 0; gi < B; ++gi) {
-      const int g = gorder[(size_t)gi];
-      nf.push_back((int)pos);
-      for (int i = gfirst[(size_t)g]; i < gfirst[(size_t)g + 1]; ++i) {
-        re[(size_t)pos] = sorted[i];
-        reorder[(size_t)pos] = sorted_in ? i : order[(size_t)i];
-        ++pos;
-      }
-    }
-    nf.push_back((int)count);
-    sorted_buf.swap(re);
-    sorted = sorted_buf.data();
-    order.swap(reorder);
-    gfirst.swap(nf);
-    sorted_in = false;
   }
   if (count > S.scap) {
     S.stests = DevAlloc(sizeof(int2) * count);
@@ -1590,6 +1575,7 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
   }
   if (groups + 1 > S.gcap) {
     S.gfirst = DevAlloc(sizeof(int) * (groups + 1));
+    S.gorder = DevAlloc(sizeof(int) * (groups + 1));
     S.gcap = groups + 1;
   }
   if (!S.revs.p) S.revs = DevAlloc(sizeof(unsigned long long) * kStatSlots);  // per-slot counters
@@ -1598,6 +1584,9 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
   if (!sorted_in)
     HIPC(hipMemcpyAsync(S.perm.p, order.data(), sizeof(int) * count, hipMemcpyHostToDevice, m->stream));
   HIPC(hipMemcpyAsync(S.gfirst.p, gfirst.data(), sizeof(int) * (groups + 1), hipMemcpyHostToDevice, m->stream));
+  if (!gorder.empty())
+    HIPC(hipMemcpyAsync(S.gorder.p, gorder.data(), sizeof(int) * gorder.size(), hipMemcpyHostToDevice, m->stream));
+  const int* go = gorder.empty() ? nullptr : S.gorder.as<int>();
   HIPC(hipMemsetAsync(S.revs.p, 0, sizeof(unsigned long long) * kStatSlots, m->stream));
   const long long slice_bytes = 8ll * m->dm.n * m->dm.dpad * M;
   const long long per_launch = std::max<long long>(1, std::min<long long>(groups, kSliceBudgetBytes / slice_bytes));
@@ -1638,7 +1627,7 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
     unsigned long long* rv = S.revs.as<unsigned long long>();
     if (!wide) hipLaunchKernelGGL(k_sliced_tests, grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices);
 #define CPIM_WIDE(MM, NIB, NB) \
-  hipLaunchKernelGGL((k_sliced_wide<MM, NIB, NB>), grid, block, lds, m->stream, net, base, tl, gf, g0, perm, fl, rv, slices)
+  hipLaunchKernelGGL((k_sliced_wide<MM, NIB, NB>), grid, block, lds, m->stream, net, base, tl, gf, go, g0, perm, fl, rv, slices)
 #define CPIM_WIDE_M(MM)                                \
   {                                                    \
     if (nb == 6) CPIM_WIDE(MM, true, 6);               \

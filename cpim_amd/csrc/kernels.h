@@ -1858,8 +1858,8 @@ constexpr int kWideWavesPerSimd = 8;
 template <int M, bool kNib, int NB>
 __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_wide(DevNet net, const u64* __restrict__ base_dom,
                               const int2* __restrict__ tests, const int* __restrict__ gfirst,
-                              long long g0, const int* __restrict__ perm, unsigned char* fail,
-                              unsigned long long* revs_out, u64* slices) {
+                              const int* __restrict__ gorder, long long g0, const int* __restrict__ perm,
+                              unsigned char* fail, unsigned long long* revs_out, u64* slices) {
   extern __shared__ __align__(16) char smem[];
   __shared__ WideCtl<M> ctl;
   const int n = net.n, W = net.W, dpad = net.dpad;
@@ -1869,7 +1869,9 @@ __global__ void __launch_bounds__(kWideMaxThreads, kWideWavesPerSimd) k_sliced_w
   const int wave = uniform_int(tid >> 6), nwaves = nthreads >> 6;
   const int B = gridDim.x, bid = blockIdx.x;
   const int xcd = bid & 7, q = B >> 3, rem = B & 7;
-  const long long g = g0 + (long long)xcd * q + (xcd < rem ? xcd : rem) + (bid >> 3);
+  // slot of this workgroup (consecutive groups on one XCD), and its group
+  const long long gs = g0 + (long long)xcd * q + (xcd < rem ? xcd : rem) + (bid >> 3);
+  const long long g = gorder ? gorder[gs] : gs;
   const int tf = gfirst[g];
   const int tc = uniform_int(gfirst[g + 1] - tf);
   u64 all[M];
