@@ -1535,57 +1535,6 @@ __device__ __forceinline__ unsigned lds_addr(const void* p) {
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) const v4u lds_v4u;
 
-// Four masks, nibbles from 0: the same lookups with the table addresses and
-// the OR-and-clear written for the gfx950 ALU.  tab must be 256-byte aligned
-// in LDS, so that the subset offset (16 bytes per subset) is ORed into the
-// base with one three-input op ((x & 0xF0) | base, truth table 0xEA) and the
-// nibble's block offset rides in the read's immediate.  Per step of four
-// lookups: 7 address ops and 16 mask ops, where the generic form takes 11
-// and 24.  Without kCheck both steps run without the early-exit tests (for
-// the first half-word of a chunk, where some test of nearly every lane still
-// lacks a support after 16 values: the tests cost 12 ops per chunk, and a
-// lane with no bits left only looks up the zero subset).
-template <bool kCheck>
-__device__ __forceinline__ void clear_supported_nib4(u64 (&pend)[4], unsigned bits, const u64* tab) {
-  const unsigned base = lds_addr(tab);
-  unsigned p[8];
-#pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    p[2 * m] = (unsigned)pend[m];
-    p[2 * m + 1] = (unsigned)(pend[m] >> 32);
-  }
-#pragma unroll
-  for (int q = 0; q < 8; q += 4) {
-    if (kCheck && (!bits || !((p[0] | p[1] | p[2] | p[3]) | (p[4] | p[5] | p[6] | p[7])))) break;
-    v4u lo[4], hi[4];
-    unsigned a[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int sh = 4 * (q + i) - 4;
-      const unsigned x = sh >= 0 ? (bits >> sh) : (bits << 4);
-      a[i] = __builtin_amdgcn_bitop3_b32(x, 0xF0u, base, 0xEA) + (q + i) * 512;
-      lo[i] = ((lds_v4u*)(size_t)a[i])[0];  // masks 0, 1 of the subset
-    }
-    p[0] = andnot_or4(p[0], lo[0].x, lo[1].x, lo[2].x, lo[3].x);
-    p[1] = andnot_or4(p[1], lo[0].y, lo[1].y, lo[2].y, lo[3].y);
-    p[2] = andnot_or4(p[2], lo[0].z, lo[1].z, lo[2].z, lo[3].z);
-    p[3] = andnot_or4(p[3], lo[0].w, lo[1].w, lo[2].w, lo[3].w);
-    // the second plane's reads after the first plane's ORs: eight reads in
-    // flight at once would hold 32 registers and spill at the 64-register
-    // budget
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) hi[i] = ((lds_v4u*)(size_t)a[i])[16];  // masks 2, 3, 256 bytes on
-    p[4] = andnot_or4(p[4], hi[0].x, hi[1].x, hi[2].x, hi[3].x);
-    p[5] = andnot_or4(p[5], hi[0].y, hi[1].y, hi[2].y, hi[3].y);
-    p[6] = andnot_or4(p[6], hi[0].z, hi[1].z, hi[2].z, hi[3].z);
-    p[7] = andnot_or4(p[7], hi[0].w, hi[1].w, hi[2].w, hi[3].w);
-    bits &= ~(0xffffu << (4 * q));
-  }
-#pragma unroll
-  for (int m = 0; m < 4; ++m) pend[m] = (u64)p[2 * m] | ((u64)p[2 * m + 1] << 32);
-}
-
 __device__ __forceinline__ void split_masks(const u64 (&pend)[4], unsigned (&p)[8]) {
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
@@ -1599,18 +1548,25 @@ __device__ __forceinline__ void join_masks(const unsigned (&p)[8], u64 (&pend)[4
   for (int m = 0; m < 4; ++m) pend[m] = (u64)p[2 * m] | ((u64)p[2 * m + 1] << 32);
 }
 
-// The lookups of clear_supported_nib4 on 32-bit halves, with the caller's
-// any-pending flag live kept current: kCheck runs a step only while live
-// and recomputes live after each step; without kCheck both steps run and
-// live is computed once at the end.
+// Four masks, nibbles from 0, on the eight 32-bit halves p of the pending
+// masks: the lookups of clear_supported_nib with the table addresses and the
+// OR-and-clear written for the gfx950 ALU.  tab must be 256-byte aligned in
+// LDS, so that the subset offset (16 bytes per subset) is ORed into the base
+// with one three-input op ((x & vF0) | base, truth table 0xEA, vF0 = 0xF0)
+// and the nibble's block offset rides in the read's immediate.  Per step of
+// four lookups: 7 address ops and 16 mask ops, where the generic form takes
+// 11 and 24.  live is the lane's any-pending flag: with kCheck a lane runs a
+// step only while live (a live lane with no bits left in this half only
+// looks up the zero subset) and live is recomputed after each step; without
+// kCheck both steps run and live is computed once at the end (for the first
+// half-word of a chunk, where some test of nearly every lane still lacks a
+// support after 16 values).
 template <bool kCheck>
 __device__ __forceinline__ void nib4_run(unsigned (&p)[8], unsigned bits, const u64* tab, bool& live,
                                          unsigned vF0) {
   const unsigned base = lds_addr(tab);
 #pragma unroll
   for (int q = 0; q < 8; q += 4) {
-    // a live lane with no bits left in this half only looks up zero
-    // subsets, so the wave tests live alone
     if (kCheck && !live) break;
     v4u lo[4], hi[4];
     unsigned a[4];
@@ -1619,15 +1575,17 @@ __device__ __forceinline__ void nib4_run(unsigned (&p)[8], unsigned bits, const 
       const int sh = 4 * (q + i) - 4;
       const unsigned x = sh >= 0 ? (bits >> sh) : (bits << 4);
       a[i] = __builtin_amdgcn_bitop3_b32(x, vF0, base, 0xEA) + (q + i) * 512;
-      lo[i] = ((lds_v4u*)(size_t)a[i])[0];
+      lo[i] = ((lds_v4u*)(size_t)a[i])[0];  // masks 0, 1 of the subset
     }
     p[0] = andnot_or4(p[0], lo[0].x, lo[1].x, lo[2].x, lo[3].x);
     p[1] = andnot_or4(p[1], lo[0].y, lo[1].y, lo[2].y, lo[3].y);
     p[2] = andnot_or4(p[2], lo[0].z, lo[1].z, lo[2].z, lo[3].z);
     p[3] = andnot_or4(p[3], lo[0].w, lo[1].w, lo[2].w, lo[3].w);
+    // the second plane's reads after the first plane's ORs: eight reads in
+    // flight at once hold 32 registers against the 64-register budget
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) hi[i] = ((lds_v4u*)(size_t)a[i])[16];
+    for (int i = 0; i < 4; ++i) hi[i] = ((lds_v4u*)(size_t)a[i])[16];  // masks 2, 3, 256 bytes on
     p[4] = andnot_or4(p[4], hi[0].x, hi[1].x, hi[2].x, hi[3].x);
     p[5] = andnot_or4(p[5], hi[0].y, hi[1].y, hi[2].y, hi[3].y);
     p[6] = andnot_or4(p[6], hi[0].z, hi[1].z, hi[2].z, hi[3].z);
@@ -1636,6 +1594,17 @@ __device__ __forceinline__ void nib4_run(unsigned (&p)[8], unsigned bits, const 
     if (kCheck) live = ((p[0] | p[1] | p[2] | p[3]) | (p[4] | p[5] | p[6] | p[7])) != 0u;
   }
   if (!kCheck) live = ((p[0] | p[1] | p[2] | p[3]) | (p[4] | p[5] | p[6] | p[7])) != 0u;
+}
+
+// nib4_run on the 64-bit pending masks (used by the byte-table variants for
+// their upper half-words).
+template <bool kCheck>
+__device__ __forceinline__ void clear_supported_nib4(u64 (&pend)[4], unsigned bits, const u64* tab) {
+  unsigned p[8];
+  split_masks(pend, p);
+  bool live = any_mask<4>(pend);
+  nib4_run<kCheck>(p, bits, tab, live, 0xF0u);
+  join_masks(p, pend);
 }
 
 template <int M, int Q0 = 0, bool kCheck = true>
