@@ -1480,9 +1480,10 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
   if (count == 0) return;
   REQUIRE(count <= INT_MAX, "too many singleton tests for one call");
   // Tests are grouped by variable: a list sorted by variable (every SAC pass)
-  // needs no sort.
-  bool sorted_in = true;
-  for (long long i = 1; i < count && sorted_in; ++i) sorted_in = tests_host[i - 1].x <= tests_host[i].x;
+  // needs no sort.  The check has no early exit, so that it vectorizes.
+  int descents = 0;
+  for (long long i = 1; i < count; ++i) descents |= tests_host[i].x < tests_host[i - 1].x;
+  bool sorted_in = descents == 0;
   std::vector<int> order;
   std::vector<int2> sorted_buf;
   const int2* sorted = tests_host;
@@ -1507,10 +1508,21 @@ void run_singleton_sliced(cpim_gmodel* m, const int2* tests_host, long long coun
   w.reserve((size_t)(count / 32 + 2));
   int open = -1;  // start of the group being filled, -1 if none
   for (long long i = 0; i < count;) {
-    long long j = i;
-    while (j < count && sorted[j].x == sorted[i].x) ++j;
-    const long long len = j - i;
+    // end j of the run of x: gallop, then bisect (the list is sorted)
     const int x = sorted[i].x;
+    long long lo = i, step = 1;
+    while (lo + step < count && sorted[lo + step].x == x) {
+      lo += step;
+      step *= 2;
+    }
+    long long hi = std::min(count, lo + step);  // sorted[hi].x != x, or hi == count
+    while (hi - lo > 1) {
+      const long long mid = lo + (hi - lo) / 2;
+      if (sorted[mid].x == x) lo = mid;
+      else hi = mid;
+    }
+    const long long j = hi;
+    const long long len = j - i;
     const long long dx = m->im.adj_off[x + 1] - m->im.adj_off[x];
     if (len > cap) {
       open = -1;
