@@ -294,6 +294,22 @@ def test_sliced_arbitrary_test_lists(gpu, case, slice_variant):
         assert (got == want[7:8]).all()
 
 
+def test_sliced_long_runs_split_across_groups(gpu, slice_variant):
+    # a sorted list (no permutation) whose variables have more tests than a
+    # workgroup holds (64, 128 or 256 by the mask count), with repeated values
+    desc = workload.random_binary(30, 64, 150, 0.75, 1701, preremove=0.3)
+    o = oracle.Oracle(desc)
+    rng = np.random.default_rng(99)
+    runs = [(0, 300), (3, 40), (7, 700), (8, 1), (29, 257)]
+    tests = np.concatenate([np.stack([np.full(k, x), rng.integers(0, 64, k)], 1) for x, k in runs]).astype(np.int32)
+    want, _ = o.singleton_tests(tests, nthreads=4)
+    with GModelAdapter.Build(desc) as m:
+        m.EnforceGAC(False)
+        got, st = m.SingletonTests(tests, mode="sliced")
+        assert st.singleton_tests == len(tests)
+        assert (got == want).all()
+
+
 def test_sliced_rejects_table_networks(gpu):
     desc = ternary(1)
     with GModelAdapter.Build(desc, GModelOptions(skip_non_binary=False)) as m:
