@@ -144,18 +144,23 @@ __device__ __forceinline__ void grid_process_constraint(const DevNet& net, u64* 
 // wipe-out (a non-empty word this AND empties), so the size update needs no
 // returned value.
 constexpr int kPairBatch = 4;
+// The asynchronous fixpoint takes eight: its waves are few (two per CU) and
+// bound by the latency of their own loads, so more rows in flight per wave
+// pay (3.6-3.75 ms against 3.85-4.1 ms on the C2 cascade), where the round
+// kernels lose occupancy at eight.
+constexpr int kAsyncPairBatch = 8;
 
 // The revisions of one batch; returns the mask of lanes whose variable (in
 // myv) lost values, the values removed in the batch, and whether some domain
 // was emptied by it.
-template <bool kCoherent>
-__device__ __forceinline__ u64 revise_pairs_w1(const DevNet& net, u64* dom, int* size, const int (&cs)[kPairBatch],
+template <bool kCoherent, int P = kPairBatch>
+__device__ __forceinline__ u64 revise_pairs_w1(const DevNet& net, u64* dom, int* size, const int (&cs)[P],
                                                int k, int& myv, int& removed_total, bool& wiped_any) {
   const int lane = lane_id();
-  int xs[kPairBatch], ys[kPairBatch];
-  long long offs[kPairBatch];
+  int xs[P], ys[P];
+  long long offs[P];
 #pragma unroll
-  for (int p = 0; p < kPairBatch; ++p) {
+  for (int p = 0; p < P; ++p) {
     const ConMeta* mp = net.con + (p < k ? cs[p] : cs[0]);
     xs[p] = mp->scope[0];
     ys[p] = mp->scope[1];
@@ -164,21 +169,21 @@ __device__ __forceinline__ u64 revise_pairs_w1(const DevNet& net, u64* dom, int*
   // lane 2p: D(x_p), lane 2p + 1: D(y_p)
   myv = -1;
 #pragma unroll
-  for (int p = 0; p < kPairBatch; ++p) {
+  for (int p = 0; p < P; ++p) {
     if (p < k && lane == 2 * p) myv = xs[p];
     if (p < k && lane == 2 * p + 1) myv = ys[p];
   }
   const u64 dl = myv >= 0 ? ld_dom<kCoherent>(&dom[myv]) : 0ull;
-  u64 r0[kPairBatch], r1[kPairBatch];
+  u64 r0[P], r1[P];
 #pragma unroll
-  for (int p = 0; p < kPairBatch; ++p) {
+  for (int p = 0; p < P; ++p) {
     const u64* plane0 = net.sup + offs[p];
     r0[p] = p < k ? plane0[lane] : 0ull;
     r1[p] = p < k ? plane0[net.dpad + lane] : 0ull;
   }
   u64 keep_l = ~0ull;  // lane 2p: new mask of x_p, lane 2p + 1: of y_p
 #pragma unroll
-  for (int p = 0; p < kPairBatch; ++p) {
+  for (int p = 0; p < P; ++p) {
     if (p >= k) break;
     const u64 dx = readlane64(dl, 2 * p), dy = readlane64(dl, 2 * p + 1);
     const u64 kx = __ballot(((dx >> lane) & 1ull) && (r0[p] & dy) != 0ull);
@@ -555,12 +560,12 @@ __global__ void k_gac_async(DevNet net, u64* dom, int* size, unsigned* F, int nf
       const int nclaimed = __popc(claimed);
       const int base_c = (chunk * 64 + l) * 32;
       if (net.W == 1) {
-        // one-word domains: binary constraints kPairBatch at a time
+        // one-word domains: binary constraints kAsyncPairBatch at a time
         claimed = (unsigned)uniform_int((int)claimed);
         while (claimed) {
-          int cs[kPairBatch] = {0, 0, 0, 0};
+          int cs[kAsyncPairBatch] = {};
           int k = 0;
-          while (claimed && k < kPairBatch) {
+          while (claimed && k < kAsyncPairBatch) {
             const int b = __ffs(claimed) - 1;
             claimed &= claimed - 1u;
             const int c = base_c + b;
@@ -569,14 +574,14 @@ __global__ void k_gac_async(DevNet net, u64* dom, int* size, unsigned* F, int nf
               continue;
             }
 #pragma unroll
-            for (int p = 0; p < kPairBatch; ++p)
+            for (int p = 0; p < kAsyncPairBatch; ++p)
               if (p == k) cs[p] = c;
             ++k;
           }
           if (k) {
             int myv = -1, removed = 0;
             bool wiped = false;
-            u64 changed = revise_pairs_w1<true>(net, dom, size, cs, k, myv, removed, wiped);
+            u64 changed = revise_pairs_w1<true, kAsyncPairBatch>(net, dom, size, cs, k, myv, removed, wiped);
             revs += 2 * k;
             dels += removed;
             if (wiped && lane == 0) atomicExch(&ctl->wipe, 1);
